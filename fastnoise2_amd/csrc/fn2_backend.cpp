@@ -1,0 +1,766 @@
+// Host side of the MI355X backend: device set-up, the compiled-program cache,
+// per-call contexts and FastNoise::Generator's batch entry points
+// (Generator.h:173-450 of the reference), which here compile the tree, launch
+// one interpreter kernel over all points and copy the result back.
+//
+// Caller buffers may be host memory (copied through a device staging buffer)
+// or device memory (hipMalloc'd: the kernel reads and writes them in place).
+// Errors never fall back to another implementation: the call fills its output
+// with NaN, returns a NaN range, records a message for fn2GetLastError() and
+// prints it once to stderr.
+#include <hip/hip_runtime.h>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "fn2_backend.h"
+#include "fn2_internal.h"
+#include "fn2_gpu.h"
+
+extern "C" {
+int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
+int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
+}
+
+namespace fn2
+{
+    namespace
+    {
+        thread_local std::string tLastError;
+
+        std::mutex gMu;
+        std::once_flag gInitOnce;
+        int gInitStatus = -1;   // 0 ok
+        std::string gInitError;
+
+        std::mutex gPrintMu;
+        std::unordered_set<std::string> gPrinted;
+
+        std::unordered_map<uint64_t, std::vector<DeviceProgram*>> gPrograms;
+        size_t gProgramCount = 0;
+        constexpr size_t kMaxCachedPrograms = 8192;
+
+        std::vector<CallContext*> gContexts;
+
+        std::mutex gTileMu;
+        std::map<std::tuple<int, int, uint32_t, uint32_t>, float*> gTileTables;
+        constexpr size_t kMaxTileTables = 1024;
+
+        void DeviceInit()
+        {
+            int n = 0;
+            hipError_t e = hipGetDeviceCount( &n );
+            if( e != hipSuccess || n <= 0 )
+            {
+                (void)hipGetLastError();
+                gInitError = "no HIP device available (the FastNoise2 MI355X backend runs on the GPU only)";
+                gInitStatus = 1;
+                return;
+            }
+            gInitStatus = 0;
+        }
+    }
+
+    void SetError( const std::string& msg )
+    {
+        tLastError = msg;
+        std::lock_guard<std::mutex> lk( gPrintMu );
+        if( gPrinted.insert( msg ).second )
+        {
+            fprintf( stderr, "FastNoise2 MI355X backend error: %s\n", msg.c_str() );
+        }
+    }
+
+    void ClearError() { tLastError.clear(); }
+    const char* LastError() { return tLastError.c_str(); }
+
+    bool EnsureDevice()
+    {
+        std::call_once( gInitOnce, DeviceInit );
+        if( gInitStatus != 0 )
+        {
+            SetError( gInitError );
+            return false;
+        }
+        return true;
+    }
+
+    bool HipOk( hipError_t e, const char* what )
+    {
+        if( e == hipSuccess ) return true;
+        SetError( std::string( what ) + ": " + hipGetErrorString( e ) );
+        return false;
+    }
+
+    FastDiv MakeFastDiv( uint32_t d )
+    {
+        // Granlund & Montgomery 1994, Figure 4.1: l = ceil(log2 d),
+        // m = floor( 2^32 (2^l - d) / d ) + 1, shifts min(l,1) and max(l-1,0)
+        FastDiv f;
+        f.d = d;
+        uint32_t l = 0;
+        while( l < 32 && ( (uint64_t)1 << l ) < d ) l++;
+        f.m = (uint32_t)( ( ( (uint64_t)1 << 32 ) * ( ( (uint64_t)1 << l ) - d ) ) / d + 1 );
+        f.s1 = l < 1 ? l : 1;
+        f.s2 = l > 0 ? l - 1 : 0;
+        return f;
+    }
+
+    const DeviceProgram* GetDeviceProgram( const Program& p )
+    {
+        std::lock_guard<std::mutex> lk( gMu );
+        auto& bucket = gPrograms[p.hash];
+        for( DeviceProgram* d : bucket )
+        {
+            if( d->resultReg == p.resultReg && d->ins.size() == p.ins.size() && d->konst.size() == p.konst.size() &&
+                memcmp( d->ins.data(), p.ins.data(), p.ins.size() * sizeof( Ins ) ) == 0 &&
+                memcmp( d->konst.data(), p.konst.data(), p.konst.size() * sizeof( float ) ) == 0 )
+            {
+                return d;
+            }
+        }
+        if( gProgramCount >= kMaxCachedPrograms )
+        {
+            SetError( "device program cache full (more than 8192 distinct compiled trees)" );
+            return nullptr;
+        }
+        // every register operand must lie inside the register file and every
+        // constant-pool window (at most 16 floats) inside the uploaded pool
+        for( const Ins& in : p.ins )
+        {
+            if( in.op == OP_END || in.op >= OP_COUNT || in.dim > 4 )
+            {
+                SetError( "compiled program has an invalid instruction" );
+                return nullptr;
+            }
+            for( int r = 0; r < 6; r++ )
+            {
+                if( in.r[r] != kNoReg && in.r[r] >= kMaxRegs )
+                {
+                    SetError( "compiled program addresses a register outside the register file" );
+                    return nullptr;
+                }
+            }
+            if( in.k > p.konst.size() )
+            {
+                SetError( "compiled program addresses constants outside its pool" );
+                return nullptr;
+            }
+        }
+        if( p.resultReg < 0 || p.resultReg >= kMaxRegs || p.ins.size() > (size_t)kMaxInstructions )
+        {
+            SetError( "compiled program is malformed" );
+            return nullptr;
+        }
+        DeviceProgram* d = new DeviceProgram;
+        d->ins = p.ins;
+        d->konst = p.konst;
+        d->resultReg = p.resultReg;
+        d->dimMask = p.dimMask;
+        d->featMask = p.featMask;
+        size_t insBytes = std::max<size_t>( 1, p.ins.size() ) * sizeof( Ins );
+        // padded so that a window starting at the last constant stays in bounds
+        size_t kBytes = ( p.konst.size() + 16 ) * sizeof( float );
+        if( !HipOk( hipMalloc( (void**)&d->dIns, insBytes ), "hipMalloc(program)" ) ||
+            !HipOk( hipMalloc( (void**)&d->dKonst, kBytes ), "hipMalloc(constants)" ) )
+        {
+            delete d;
+            return nullptr;
+        }
+        if( !p.ins.empty() && !HipOk( hipMemcpy( d->dIns, p.ins.data(), p.ins.size() * sizeof( Ins ), hipMemcpyHostToDevice ), "upload program" ) ) return nullptr;
+        std::vector<float> padded( p.konst );
+        padded.resize( p.konst.size() + 16, 0.0f );
+        if( !HipOk( hipMemcpy( d->dKonst, padded.data(), padded.size() * sizeof( float ), hipMemcpyHostToDevice ), "upload constants" ) ) return nullptr;
+        bucket.push_back( d );
+        gProgramCount++;
+        return d;
+    }
+
+    bool CompileForDevice( const FastNoise::Generator* root, int dims, const DeviceProgram*& dp, Program* hostCopy )
+    {
+        Program p;
+        std::string err;
+        if( !CompileTree( root, dims, p, err ) )
+        {
+            SetError( "cannot compile node tree: " + err );
+            return false;
+        }
+        dp = GetDeviceProgram( p );
+        if( hostCopy ) *hostCopy = std::move( p );
+        return dp != nullptr;
+    }
+
+    CallContext* AcquireContext()
+    {
+        {
+            std::lock_guard<std::mutex> lk( gMu );
+            if( !gContexts.empty() )
+            {
+                CallContext* c = gContexts.back();
+                gContexts.pop_back();
+                return c;
+            }
+        }
+        CallContext* c = new CallContext;
+        if( !HipOk( hipStreamCreateWithFlags( &c->stream, hipStreamNonBlocking ), "hipStreamCreate" ) ||
+            !HipOk( hipMalloc( (void**)&c->dKeys, 2 * sizeof( int32_t ) ), "hipMalloc(min/max)" ) ||
+            !HipOk( hipHostMalloc( (void**)&c->hKeys, 2 * sizeof( int32_t ), hipHostMallocDefault ), "hipHostMalloc" ) ||
+            !HipOk( hipEventCreate( &c->ev0 ), "hipEventCreate" ) || !HipOk( hipEventCreate( &c->ev1 ), "hipEventCreate" ) )
+        {
+            return nullptr;
+        }
+        return c;
+    }
+
+    void ReleaseContext( CallContext* c )
+    {
+        if( !c ) return;
+        std::lock_guard<std::mutex> lk( gMu );
+        gContexts.push_back( c );
+    }
+
+    bool CallContext::Reserve( float*& buf, size_t& cap, size_t floats )
+    {
+        if( floats <= cap ) return true;
+        if( buf )
+        {
+            (void)hipStreamSynchronize( stream );
+            (void)hipFree( buf );
+            buf = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>( floats, 1024 );
+        if( !HipOk( hipMalloc( (void**)&buf, want * sizeof( float ) ), "hipMalloc(staging)" ) ) return false;
+        cap = want;
+        return true;
+    }
+
+    bool IsDevicePointer( const void* p )
+    {
+        if( !p ) return false;
+        hipPointerAttribute_t attr;
+        hipError_t e = hipPointerGetAttributes( &attr, p );
+        if( e != hipSuccess )
+        {
+            (void)hipGetLastError();
+            return false;
+        }
+        return attr.type == hipMemoryTypeDevice;
+    }
+
+    float DecodeKey( int32_t k )
+    {
+        int32_t b = k >= 0 ? k : k ^ 0x7fffffff;
+        float f;
+        memcpy( &f, &b, 4 );
+        return f;
+    }
+
+    // Points a spec describes (0 when a count is negative or the product overflows int)
+    int64_t SpecPoints( const fn2GenSpec& s )
+    {
+        int64_t n = 1;
+        int axes = s.kind == FN2_SOURCE_POSITIONS ? 1 : s.kind == FN2_SOURCE_TILEABLE ? 2 : s.dims;
+        if( axes < 1 || axes > 4 ) return 0;
+        for( int d = 0; d < axes; d++ )
+        {
+            if( s.count[d] < 0 ) return 0;
+            n *= s.count[d];
+        }
+        return n > 0x7fffffff ? 0 : n;
+    }
+
+    // GenTileable2D maps column x and row y onto a 4D torus (Generator.inl:370-433
+    // of the reference): x' = cos( x * xMul ) * xFreq, z' = sin( x * xMul ) * xFreq
+    // and the same for y with w'. Those coordinates depend on the column or the
+    // row alone, so they are tabulated once per (size, step) on the host and
+    // kept on the device: [cos x | sin x] (2 * xSize), then [cos y | sin y].
+    const float* TileTable( int xSize, int ySize, float xStep, float yStep, std::string& err )
+    {
+        uint32_t sx, sy;
+        memcpy( &sx, &xStep, 4 );
+        memcpy( &sy, &yStep, 4 );
+        auto key = std::make_tuple( xSize, ySize, sx, sy );
+        std::lock_guard<std::mutex> lk( gTileMu );
+        auto it = gTileTables.find( key );
+        if( it != gTileTables.end() ) return it->second;
+        if( gTileTables.size() >= kMaxTileTables )
+        {
+            err = "more than 1024 distinct tileable sizes in one process";
+            return nullptr;
+        }
+        const float pi2Recip = 0.15915493667f;
+        const float xSizePi = (float)xSize * pi2Recip;
+        const float ySizePi = (float)ySize * pi2Recip;
+        const float xFreq = xSizePi * xStep;
+        const float yFreq = ySizePi * yStep;
+        const float xMul = 1 / xSizePi;
+        const float yMul = 1 / ySizePi;
+        std::vector<float> t( 2 * ( (size_t)xSize + ySize ) );
+        for( int i = 0; i < xSize; i++ )
+        {
+            float f = (float)i * xMul;
+            t[i] = cosf( f ) * xFreq;
+            t[xSize + i] = sinf( f ) * xFreq;
+        }
+        for( int j = 0; j < ySize; j++ )
+        {
+            float f = (float)j * yMul;
+            t[2 * xSize + j] = cosf( f ) * yFreq;
+            t[2 * xSize + ySize + j] = sinf( f ) * yFreq;
+        }
+        float* d = nullptr;
+        hipError_t e = hipMalloc( (void**)&d, t.size() * sizeof( float ) );
+        if( e == hipSuccess ) e = hipMemcpy( d, t.data(), t.size() * sizeof( float ), hipMemcpyHostToDevice );
+        if( e != hipSuccess )
+        {
+            err = std::string( "tileable table upload: " ) + hipGetErrorString( e );
+            return nullptr;
+        }
+        gTileTables.emplace( key, d );
+        return d;
+    }
+
+    bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err )
+    {
+        a.kind = s.kind;
+        a.dims = s.dims;
+        a.seed = s.seed;
+        if( s.kind == FN2_SOURCE_TILEABLE )
+        {
+            if( s.dims != 4 && s.dims != 2 ) { err = "tileable generation is 2D"; return false; }
+            a.dims = 4;
+            if( s.count[0] < 0 || s.count[1] < 0 ) { err = "negative size"; return false; }
+            total = (int64_t)s.count[0] * s.count[1];
+            if( total > 0 )
+            {
+                a.div[0] = MakeFastDiv( (uint32_t)s.count[0] );
+                const float* tab = TileTable( s.count[0], s.count[1], s.step[0], s.step[1], err );
+                if( !tab ) return false;
+                a.pos[0] = tab;
+                a.pos[1] = tab + 2 * s.count[0];
+                a.pos[2] = tab + s.count[0];
+                a.pos[3] = tab + 2 * s.count[0] + s.count[1];
+            }
+        }
+        else if( s.kind == FN2_SOURCE_GRID )
+        {
+            if( s.dims < 2 || s.dims > 4 ) { err = "grid dimensions must be 2, 3 or 4"; return false; }
+            total = 1;
+            for( int d = 0; d < s.dims; d++ )
+            {
+                if( s.count[d] < 0 ) { err = "negative count"; return false; }
+                total *= s.count[d];
+                a.offset[d] = s.offset[d];
+                a.step[d] = s.step[d];
+            }
+            if( total > 0 )
+            {
+                for( int d = 0; d < s.dims - 1; d++ ) a.div[d] = MakeFastDiv( (uint32_t)s.count[d] );
+            }
+        }
+        else if( s.kind == FN2_SOURCE_POSITIONS )
+        {
+            if( s.dims < 2 || s.dims > 4 ) { err = "position dimensions must be 2, 3 or 4"; return false; }
+            if( s.count[0] < 0 ) { err = "negative count"; return false; }
+            total = s.count[0];
+            for( int d = 0; d < s.dims; d++ )
+            {
+                a.offset[d] = s.offset[d];
+                a.pos[d] = s.pos[d];
+            }
+        }
+        else
+        {
+            err = "unknown source kind";
+            return false;
+        }
+        if( total > 0x7fffffff )
+        {
+            err = "more than 2^31-1 points in one call";
+            return false;
+        }
+        a.total = (int32_t)total;
+        return true;
+    }
+
+    // One batch call of the public API: host or device buffers, synchronous.
+    OutputRange GenerateHost( const FastNoise::Generator* root, const fn2GenSpec& spec, float* out, const float* const* hostPos )
+    {
+        ClearError();
+        OutputRange r;
+        GenArgs a{};
+        int64_t total = 0;
+        std::string err;
+        auto fail = [&]( const std::string& msg ) -> OutputRange
+        {
+            SetError( msg );
+            if( out && total > 0 && !IsDevicePointer( out ) )
+            {
+                for( int64_t i = 0; i < total; i++ ) out[i] = __builtin_nanf( "" );
+            }
+            OutputRange bad;
+            bad.min = bad.max = __builtin_nanf( "" );
+            bad.ok = false;
+            return bad;
+        };
+        total = SpecPoints( spec );
+        if( !EnsureDevice() ) return fail( gInitError );
+        if( !FillSpecArgs( spec, a, total, err ) ) return fail( err );
+        if( total == 0 ) return r;
+        if( !out ) return fail( "null output buffer" );
+
+        const DeviceProgram* dp = nullptr;
+        if( !CompileForDevice( root, a.dims, dp, nullptr ) ) return fail( tLastError );
+
+        CallContext* c = AcquireContext();
+        if( !c ) return fail( tLastError );
+        struct Releaser { CallContext* c; ~Releaser() { ReleaseContext( c ); } } rel{ c };
+
+        a.ins = dp->dIns;
+        a.konst = dp->dKonst;
+        a.nIns = (int32_t)dp->ins.size();
+        a.resultReg = dp->resultReg;
+        a.minmax = c->dKeys;
+
+        const bool devOut = IsDevicePointer( out );
+        if( devOut ) a.out = out;
+        else
+        {
+            if( !c->Reserve( c->dOut, c->outCap, (size_t)total ) ) return fail( tLastError );
+            a.out = c->dOut;
+        }
+        if( spec.kind == FN2_SOURCE_POSITIONS )
+        {
+            bool allDev = true;
+            for( int d = 0; d < spec.dims; d++ ) allDev = allDev && IsDevicePointer( hostPos[d] );
+            if( allDev )
+            {
+                for( int d = 0; d < spec.dims; d++ ) a.pos[d] = hostPos[d];
+            }
+            else
+            {
+                if( !c->Reserve( c->dPos, c->posCap, (size_t)total * spec.dims ) ) return fail( tLastError );
+                for( int d = 0; d < spec.dims; d++ )
+                {
+                    if( !hostPos[d] ) return fail( "null position array" );
+                    float* dst = c->dPos + (size_t)d * total;
+                    if( !HipOk( hipMemcpyAsync( dst, hostPos[d], (size_t)total * sizeof( float ), hipMemcpyDefault, c->stream ), "copy positions" ) ) return fail( tLastError );
+                    a.pos[d] = dst;
+                }
+            }
+        }
+
+        if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, c->stream ), "launch min/max init" ) ) return fail( tLastError );
+        if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, c->stream ), "launch generator" ) ) return fail( tLastError );
+        if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, 2 * sizeof( int32_t ), hipMemcpyDeviceToHost, c->stream ), "read min/max" ) ) return fail( tLastError );
+        if( !devOut )
+        {
+            if( !HipOk( hipMemcpyAsync( out, a.out, (size_t)total * sizeof( float ), hipMemcpyDeviceToHost, c->stream ), "copy output" ) ) return fail( tLastError );
+        }
+        if( !HipOk( hipStreamSynchronize( c->stream ), "generator kernel" ) ) return fail( tLastError );
+
+        if( c->hKeys[0] == 0x7fffffff && c->hKeys[1] == (int32_t)0x80000000u )
+        {
+            r.min = FastNoise::kInfinity;
+            r.max = -FastNoise::kInfinity;
+        }
+        else
+        {
+            r.min = DecodeKey( c->hKeys[0] );
+            r.max = DecodeKey( c->hKeys[1] );
+        }
+        return r;
+    }
+}
+
+using namespace fn2;
+
+namespace FastNoise
+{
+    static OutputMinMax ToMinMax( const OutputRange& r )
+    {
+        OutputMinMax m;
+        m.min = r.min;
+        m.max = r.max;
+        return m;
+    }
+
+    // Generator.h:217-221 of the reference
+    OutputMinMax Generator::GenUniformGrid2D( float* out, float xOffset, float yOffset, int xCount, int yCount, float xStepSize, float yStepSize, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_GRID; s.dims = 2; s.seed = seed;
+        s.count[0] = xCount; s.count[1] = yCount;
+        s.offset[0] = xOffset; s.offset[1] = yOffset;
+        s.step[0] = xStepSize; s.step[1] = yStepSize;
+        return ToMinMax( GenerateHost( this, s, out, nullptr ) );
+    }
+
+    // Generator.h:240-244 of the reference
+    OutputMinMax Generator::GenUniformGrid3D( float* out, float xOffset, float yOffset, float zOffset, int xCount, int yCount, int zCount, float xStepSize, float yStepSize, float zStepSize, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_GRID; s.dims = 3; s.seed = seed;
+        s.count[0] = xCount; s.count[1] = yCount; s.count[2] = zCount;
+        s.offset[0] = xOffset; s.offset[1] = yOffset; s.offset[2] = zOffset;
+        s.step[0] = xStepSize; s.step[1] = yStepSize; s.step[2] = zStepSize;
+        return ToMinMax( GenerateHost( this, s, out, nullptr ) );
+    }
+
+    // Generator.h:266-270 of the reference
+    OutputMinMax Generator::GenUniformGrid4D( float* out, float xOffset, float yOffset, float zOffset, float wOffset, int xCount, int yCount, int zCount, int wCount, float xStepSize, float yStepSize, float zStepSize, float wStepSize, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_GRID; s.dims = 4; s.seed = seed;
+        s.count[0] = xCount; s.count[1] = yCount; s.count[2] = zCount; s.count[3] = wCount;
+        s.offset[0] = xOffset; s.offset[1] = yOffset; s.offset[2] = zOffset; s.offset[3] = wOffset;
+        s.step[0] = xStepSize; s.step[1] = yStepSize; s.step[2] = zStepSize; s.step[3] = wStepSize;
+        return ToMinMax( GenerateHost( this, s, out, nullptr ) );
+    }
+
+    // Generator.h:285-287 of the reference
+    OutputMinMax Generator::GenTileable2D( float* out, int xSize, int ySize, float xStepSize, float yStepSize, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_TILEABLE; s.dims = 2; s.seed = seed;
+        s.count[0] = xSize; s.count[1] = ySize;
+        s.step[0] = xStepSize; s.step[1] = yStepSize;
+        return ToMinMax( GenerateHost( this, s, out, nullptr ) );
+    }
+
+    // Generator.h:301-304 of the reference
+    OutputMinMax Generator::GenPositionArray2D( float* out, int count, const float* xPosArray, const float* yPosArray, float xOffset, float yOffset, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_POSITIONS; s.dims = 2; s.seed = seed;
+        s.count[0] = count;
+        s.offset[0] = xOffset; s.offset[1] = yOffset;
+        const float* p[4] = { xPosArray, yPosArray, nullptr, nullptr };
+        return ToMinMax( GenerateHost( this, s, out, p ) );
+    }
+
+    // Generator.h:320-323 of the reference
+    OutputMinMax Generator::GenPositionArray3D( float* out, int count, const float* xPosArray, const float* yPosArray, const float* zPosArray, float xOffset, float yOffset, float zOffset, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_POSITIONS; s.dims = 3; s.seed = seed;
+        s.count[0] = count;
+        s.offset[0] = xOffset; s.offset[1] = yOffset; s.offset[2] = zOffset;
+        const float* p[4] = { xPosArray, yPosArray, zPosArray, nullptr };
+        return ToMinMax( GenerateHost( this, s, out, p ) );
+    }
+
+    // Generator.h:342-345 of the reference
+    OutputMinMax Generator::GenPositionArray4D( float* out, int count, const float* xPosArray, const float* yPosArray, const float* zPosArray, const float* wPosArray, float xOffset, float yOffset, float zOffset, float wOffset, int seed ) const
+    {
+        fn2GenSpec s{};
+        s.kind = FN2_SOURCE_POSITIONS; s.dims = 4; s.seed = seed;
+        s.count[0] = count;
+        s.offset[0] = xOffset; s.offset[1] = yOffset; s.offset[2] = zOffset; s.offset[3] = wOffset;
+        const float* p[4] = { xPosArray, yPosArray, zPosArray, wPosArray };
+        return ToMinMax( GenerateHost( this, s, out, p ) );
+    }
+
+    // GenSingle*: one point through the position path. The offset is -0.0f,
+    // the additive identity, so the point reaches the tree bit for bit
+    // (Generator.h:351-368 of the reference).
+    float Generator::GenSingle2D( float x, float y, int seed ) const
+    {
+        float v = 0;
+        GenPositionArray2D( &v, 1, &x, &y, -0.0f, -0.0f, seed );
+        return v;
+    }
+
+    float Generator::GenSingle3D( float x, float y, float z, int seed ) const
+    {
+        float v = 0;
+        GenPositionArray3D( &v, 1, &x, &y, &z, -0.0f, -0.0f, -0.0f, seed );
+        return v;
+    }
+
+    float Generator::GenSingle4D( float x, float y, float z, float w, int seed ) const
+    {
+        float v = 0;
+        GenPositionArray4D( &v, 1, &x, &y, &z, &w, -0.0f, -0.0f, -0.0f, -0.0f, seed );
+        return v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-side extension API (fn2_gpu.h)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+FASTNOISE_API const char* fn2GetLastError( void )
+{
+    return fn2::LastError();
+}
+
+FASTNOISE_API int fn2DeviceCount( void )
+{
+    int n = 0;
+    if( hipGetDeviceCount( &n ) != hipSuccess )
+    {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+FASTNOISE_API void* fn2Malloc( size_t bytes )
+{
+    ClearError();
+    if( !EnsureDevice() ) return nullptr;
+    void* p = nullptr;
+    if( !HipOk( hipMalloc( &p, bytes ), "hipMalloc" ) ) return nullptr;
+    return p;
+}
+
+FASTNOISE_API void fn2Free( void* p )
+{
+    if( p ) (void)hipFree( p );
+}
+
+FASTNOISE_API int fn2Memcpy( void* dst, const void* src, size_t bytes, int kind )
+{
+    ClearError();
+    if( !EnsureDevice() ) return 1;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    return HipOk( hipMemcpy( dst, src, bytes, k ), "hipMemcpy" ) ? 0 : 1;
+}
+
+FASTNOISE_API int fn2DeviceSynchronize( void )
+{
+    ClearError();
+    if( !EnsureDevice() ) return 1;
+    return HipOk( hipDeviceSynchronize(), "hipDeviceSynchronize" ) ? 0 : 1;
+}
+
+FASTNOISE_API int fn2GetProgramInfo( const void* node, int dims, int* instructions, int* registers, int* dimMask, int* featureMask )
+{
+    ClearError();
+    if( !node ) { SetError( "null node" ); return 1; }
+    Program p;
+    std::string err;
+    if( !CompileTree( static_cast<const FastNoise::Generator*>( node ), dims, p, err ) )
+    {
+        SetError( "cannot compile node tree: " + err );
+        return 1;
+    }
+    if( instructions ) *instructions = (int)p.ins.size();
+    if( registers ) *registers = p.regsUsed;
+    if( dimMask ) *dimMask = p.dimMask;
+    if( featureMask ) *featureMask = p.featMask;
+    return 0;
+}
+
+FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, float* dOut, float* outputMinMax, void* stream )
+{
+    ClearError();
+    if( !node || !spec ) { SetError( "null argument" ); return 1; }
+    GenArgs a{};
+    int64_t total = 0;
+    std::string err;
+    if( !EnsureDevice() ) return 1;
+    if( !FillSpecArgs( *spec, a, total, err ) ) { SetError( err ); return 1; }
+    if( total == 0 )
+    {
+        if( outputMinMax ) { outputMinMax[0] = FastNoise::kInfinity; outputMinMax[1] = -FastNoise::kInfinity; }
+        return 0;
+    }
+    const DeviceProgram* dp = nullptr;
+    if( !CompileForDevice( static_cast<const FastNoise::Generator*>( node ), a.dims, dp, nullptr ) ) return 1;
+    CallContext* c = AcquireContext();
+    if( !c ) return 1;
+    struct Releaser { CallContext* c; ~Releaser() { ReleaseContext( c ); } } rel{ c };
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    a.ins = dp->dIns;
+    a.konst = dp->dKonst;
+    a.nIns = (int32_t)dp->ins.size();
+    a.resultReg = dp->resultReg;
+    a.out = dOut;
+    a.minmax = outputMinMax ? c->dKeys : nullptr;
+    if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, st ), "launch min/max init" ) ) return 1;
+    if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, st ), "launch generator" ) ) return 1;
+    if( outputMinMax )
+    {
+        if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, 2 * sizeof( int32_t ), hipMemcpyDeviceToHost, st ), "read min/max" ) ) return 1;
+        if( !HipOk( hipStreamSynchronize( st ), "generator kernel" ) ) return 1;
+        if( c->hKeys[0] == 0x7fffffff && c->hKeys[1] == (int32_t)0x80000000u )
+        {
+            outputMinMax[0] = FastNoise::kInfinity;
+            outputMinMax[1] = -FastNoise::kInfinity;
+        }
+        else
+        {
+            outputMinMax[0] = DecodeKey( c->hKeys[0] );
+            outputMinMax[1] = DecodeKey( c->hKeys[1] );
+        }
+    }
+    else if( !stream )
+    {
+        // the library stream belongs to the context that is released below:
+        // finish the work before another call can pick it up
+        if( !HipOk( hipStreamSynchronize( st ), "generator kernel" ) ) return 1;
+    }
+    return 0;
+}
+
+FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, float* dOut, int warmup, int iters, double* msPerLaunch, double* wallMsPerLaunch )
+{
+    ClearError();
+    if( !node || !spec || iters <= 0 ) { SetError( "bad argument" ); return 1; }
+    GenArgs a{};
+    int64_t total = 0;
+    std::string err;
+    if( !EnsureDevice() ) return 1;
+    if( !FillSpecArgs( *spec, a, total, err ) ) { SetError( err ); return 1; }
+    const DeviceProgram* dp = nullptr;
+    if( !CompileForDevice( static_cast<const FastNoise::Generator*>( node ), a.dims, dp, nullptr ) ) return 1;
+    CallContext* c = AcquireContext();
+    if( !c ) return 1;
+    struct Releaser { CallContext* c; ~Releaser() { ReleaseContext( c ); } } rel{ c };
+    a.ins = dp->dIns;
+    a.konst = dp->dKonst;
+    a.nIns = (int32_t)dp->ins.size();
+    a.resultReg = dp->resultReg;
+    a.out = dOut;
+    a.minmax = c->dKeys;
+    for( int i = 0; i < warmup; i++ )
+    {
+        if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, c->stream ), "launch" ) ) return 1;
+        if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, c->stream ), "launch" ) ) return 1;
+    }
+    if( !HipOk( hipStreamSynchronize( c->stream ), "warmup" ) ) return 1;
+    // The range keys are reset once: every timed launch writes the same
+    // output, so the keys accumulate the same min/max, and the timed region
+    // holds generator launches only.
+    if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, c->stream ), "launch" ) ) return 1;
+    if( !HipOk( hipStreamSynchronize( c->stream ), "range reset" ) ) return 1;
+    auto t0 = std::chrono::steady_clock::now();
+    if( !HipOk( hipEventRecord( c->ev0, c->stream ), "event" ) ) return 1;
+    for( int i = 0; i < iters; i++ )
+    {
+        if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, c->stream ), "launch" ) ) return 1;
+    }
+    if( !HipOk( hipEventRecord( c->ev1, c->stream ), "event" ) ) return 1;
+    if( !HipOk( hipStreamSynchronize( c->stream ), "timed launches" ) ) return 1;
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0;
+    if( !HipOk( hipEventElapsedTime( &ms, c->ev0, c->ev1 ), "hipEventElapsedTime" ) ) return 1;
+    if( msPerLaunch ) *msPerLaunch = ms / iters;
+    if( wallMsPerLaunch ) *wallMsPerLaunch = std::chrono::duration<double, std::milli>( t1 - t0 ).count() / iters;
+    return 0;
+}
+
+}

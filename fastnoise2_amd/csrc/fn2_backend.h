@@ -1,0 +1,67 @@
+// Shared pieces of the host backend (fn2_backend.cpp) used by the C API and
+// the serving engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "fn2_compiler.h"
+#include "fn2_kernel_args.h"
+#include "fn2_gpu.h"
+
+namespace FastNoise { class Generator; }
+
+namespace fn2
+{
+    // A compiled program resident in device memory (kept for the process).
+    struct DeviceProgram
+    {
+        std::vector<Ins> ins;
+        std::vector<float> konst;
+        int resultReg = 0;
+        int dimMask = 0;
+        int featMask = 0;
+        Ins* dIns = nullptr;
+        float* dKonst = nullptr;
+    };
+
+    // Per-call stream and staging buffers, pooled.
+    struct CallContext
+    {
+        hipStream_t stream = nullptr;
+        float* dOut = nullptr;
+        size_t outCap = 0;
+        float* dPos = nullptr;
+        size_t posCap = 0;
+        int32_t* dKeys = nullptr;
+        int32_t* hKeys = nullptr;   // pinned
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+        bool Reserve( float*& buf, size_t& cap, size_t floats );
+    };
+
+    struct OutputRange
+    {
+        float min = __builtin_inff();
+        float max = -__builtin_inff();
+        bool ok = true;
+    };
+
+    void SetError( const std::string& msg );
+    void ClearError();
+    const char* LastError();
+    bool EnsureDevice();
+    bool HipOk( hipError_t e, const char* what );
+
+    FastDiv MakeFastDiv( uint32_t d );
+    float DecodeKey( int32_t k );
+    bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err );
+
+    const DeviceProgram* GetDeviceProgram( const Program& p );
+    bool CompileForDevice( const FastNoise::Generator* root, int dims, const DeviceProgram*& dp, Program* hostCopy );
+
+    CallContext* AcquireContext();
+    void ReleaseContext( CallContext* c );
+    bool IsDevicePointer( const void* p );
+
+    OutputRange GenerateHost( const FastNoise::Generator* root, const fn2GenSpec& spec, float* out, const float* const* hostPos );
+}

@@ -1,0 +1,500 @@
+// The node-program interpreter of the generator kernels: per-point start
+// position (uniform grid, position arrays or the tileable 4D torus of
+// Generator.inl:79-433) and the wave-uniform instruction loop over a
+// 32-entry register file (uniform indices -> GPR indexing, no scratch).
+// Host-and-device code: the kernels in fn2_kernels.hip instantiate it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fn2_program.h"
+#include "fn2_device_cellwarp.h"
+#include "fn2_kernel_args.h"
+
+namespace fn2i
+{
+    using namespace fn2d;
+    using fn2::Ins;
+
+    FN2D uint32_t fastDiv( const fn2::FastDiv& d, uint32_t n )
+    {
+        uint32_t t1 = (uint32_t)( ( (uint64_t)d.m * n ) >> 32 );
+        return ( t1 + ( ( n - t1 ) >> d.s1 ) ) >> d.s2;
+    }
+
+    // order-preserving int key of a float (for atomicMin / atomicMax)
+    FN2D int32_t orderedKey( float f )
+    {
+        int32_t b = __builtin_bit_cast( int32_t, f );
+        return b >= 0 ? b : b ^ 0x7fffffff;
+    }
+
+    template<int D>
+    FN2D void leafAt( const Ins& in, const float* K, int32_t seed, const float* R, int p, float& out )
+    {
+        const float freq = K[0];
+        float v;
+        switch( in.op )
+        {
+        case fn2::OP_PERLIN:
+            if constexpr( D == 2 ) v = perlin2( seed, R[p] * freq, R[p + 1] * freq );
+            else if constexpr( D == 3 ) v = perlin3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
+            else v = perlin4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
+            break;
+        case fn2::OP_VALUE:
+            if constexpr( D == 2 ) v = value2( seed, R[p] * freq, R[p + 1] * freq );
+            else if constexpr( D == 3 ) v = value3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
+            else v = value4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
+            break;
+        case fn2::OP_SIMPLEX:
+            if constexpr( D == 2 ) v = simplex2( seed, R[p] * freq, R[p + 1] * freq );
+            else if constexpr( D == 3 ) v = simplex3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
+            else v = simplex4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
+            break;
+        default:
+            if constexpr( D == 2 ) v = superSimplex2( seed, R[p] * freq, R[p + 1] * freq );
+            else if constexpr( D == 3 ) v = superSimplex3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
+            else v = superSimplex4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
+            break;
+        }
+        out = scaleOutput( v, K[1], K[2], K[3] );
+    }
+
+    // Lattice noise on explicit coordinates (used by the fused fractal loop)
+    template<int D>
+    FN2D float leafRaw( int op, int32_t seed, const float* x )
+    {
+        switch( op )
+        {
+        case fn2::OP_PERLIN:
+            if constexpr( D == 2 ) return perlin2( seed, x[0], x[1] );
+            else if constexpr( D == 3 ) return perlin3( seed, x[0], x[1], x[2] );
+            else return perlin4( seed, x[0], x[1], x[2], x[3] );
+        case fn2::OP_VALUE:
+            if constexpr( D == 2 ) return value2( seed, x[0], x[1] );
+            else if constexpr( D == 3 ) return value3( seed, x[0], x[1], x[2] );
+            else return value4( seed, x[0], x[1], x[2], x[3] );
+        case fn2::OP_SIMPLEX:
+            if constexpr( D == 2 ) return simplex2( seed, x[0], x[1] );
+            else if constexpr( D == 3 ) return simplex3( seed, x[0], x[1], x[2] );
+            else return simplex4( seed, x[0], x[1], x[2], x[3] );
+        default:
+            if constexpr( D == 2 ) return superSimplex2( seed, x[0], x[1] );
+            else if constexpr( D == 3 ) return superSimplex3( seed, x[0], x[1], x[2] );
+            else return superSimplex4( seed, x[0], x[1], x[2], x[3] );
+        }
+    }
+
+    // FractalFBm / FractalRidged over a lattice leaf with constant gain and
+    // weighted strength (Fractal.inl:9-67); the octave loop stays in registers.
+    template<int D>
+    FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, const float* R, int p )
+    {
+        const int leaf = in.p[0], octaves = in.p[1];
+        const bool ridged = in.p[2] != 0;
+        const float freq = K[0], nMin = K[1], oScale = K[2], rMin = K[3];
+        const float gain = K[4], ws = K[5], lac = K[6];
+        float pos[D];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) pos[a] = R[p + a];
+        auto octave = [&]( int32_t s ) {
+            float x[D];
+#pragma unroll
+            for( int a = 0; a < D; a++ ) x[a] = pos[a] * freq;
+            return scaleOutput( leafRaw<D>( leaf, s, x ), nMin, oScale, rMin );
+        };
+        float amp = 1.0f;
+        float noise = octave( seed );
+        if( ridged ) noise = fabs_( noise );
+        float sum = ridged ? ( noise * -2.0f + 1.0f ) * amp : noise * amp;
+        for( int i = 1; i < octaves; i++ )
+        {
+            seed = iadd( seed, 1 );
+            amp *= ridged ? lerp( 1.0f, 1.0f - noise, ws ) : lerp( 1.0f, ( noise + 1.0f ) * 0.5f, ws );
+            amp *= gain;
+#pragma unroll
+            for( int a = 0; a < D; a++ ) pos[a] = pos[a] * lac;
+            noise = octave( seed );
+            if( ridged )
+            {
+                noise = fabs_( noise );
+                sum += ( noise * -2.0f + 1.0f ) * amp;
+            }
+            else sum += noise * amp;
+        }
+        return sum;
+    }
+
+    template<int D>
+    FN2D void cellular( const Ins& in, const float* K, int32_t seed, float* R )
+    {
+        CellParams cp;
+        cp.seed = seed;
+        cp.jitter = jitterConst<D>() * R[in.r[2]];
+        cp.sizeJitterOn = in.r[3] != fn2::kNoReg;
+        cp.sizeJitter = cp.sizeJitterOn ? R[in.r[3]] * ( D == 2 ? -1.f / 0x3ff : D == 3 ? -1.f / 0xffff : -1.f / 0xfffff ) : 0.0f;
+        cp.distFunc = in.p[0];
+        cp.minkowskiP = K[3];
+        const float freq = K[0];
+        float pos[D];
+        const int p = in.r[1];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) pos[a] = R[p + a] * freq;
+        if( in.op == fn2::OP_CELL_VALUE )
+        {
+            R[in.r[0]] = scaleOutput( cellularValue<D>( cp, pos, in.p[1] ), K[1], K[2], K[4] );
+        }
+        else if( in.op == fn2::OP_CELL_DISTANCE )
+        {
+            float dist[4];
+            cellularDistances<D>( cp, pos, dist );
+            R[in.r[0]] = scaleOutput( cellularReturn( dist, in.p[0], in.p[1], in.p[2], in.p[3], 0.0f ), K[1], K[2], K[4] );
+        }
+        else
+        {
+            float cell[D];
+            cellularLookupPos<D>( cp, pos, cell );
+            const float scale = K[5];
+#pragma unroll
+            for( int a = 0; a < D; a++ ) R[in.r[0] + a] = cell[a] * scale;
+        }
+    }
+
+    template<int D>
+    FN2D void warpOp( const Ins& in, const float* K, int32_t seed, float* R )
+    {
+        float sp[D], out[D], axis[D];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) { sp[a] = R[in.r[1] + a]; out[a] = R[in.r[2] + a]; axis[a] = K[a]; }
+        const float amp = R[in.r[3]];
+        float s;
+        if( in.op == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );
+        else if( in.op == fn2::OP_WARP_SIMPLEX ) s = warpSimplex<D>( seed, in.p[0], amp, axis, sp, out );
+        else s = warpSuperSimplex<D>( seed, in.p[0], amp, axis, sp, out );
+#pragma unroll
+        for( int a = 0; a < D; a++ ) R[in.r[2] + a] = out[a];
+        if( in.r[0] != fn2::kNoReg ) R[in.r[0]] = s;
+    }
+
+    template<int D>
+    FN2D void distanceOp( const Ins& in, float* R )
+    {
+        float d[D];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) d[a] = R[in.r[1] + a];
+        float mp = in.r[2] != fn2::kNoReg ? R[in.r[2]] : 1.5f;
+        R[in.r[0]] = calcDistance<D>( in.p[0], in.p[1] != 0, d, mp );
+    }
+
+    // Runs the program on one point. R[0..dims) hold the starting position.
+    // DM: bit d set = d-dimensional ops compiled in; FT: feature groups
+    // compiled in (kFeatCell, kFeatWarp, kFeatMisc). The host picks the
+    // smallest variant that covers the program, which keeps the VGPR budget
+    // (and with it occupancy) of lattice-only programs low.
+    template<int DM, int FT>
+    FN2D float runProgram( const fn2::GenArgs& a, float* R )
+    {
+        for( int pc = 0; pc < a.nIns; pc++ )
+        {
+            const Ins in = a.ins[pc];
+            const float* K = a.konst + in.k;
+            const int32_t seed = iadd( a.seed, in.seed );
+            const int d = in.dim;
+            switch( in.op )
+            {
+            case fn2::OP_PERLIN: case fn2::OP_VALUE: case fn2::OP_SIMPLEX: case fn2::OP_SUPERSIMPLEX:
+            {
+                float v = 0.0f;
+                if( ( DM & 4 ) && d == 2 ) leafAt<2>( in, K, seed, R, in.r[1], v );
+                else if( ( DM & 8 ) && d == 3 ) leafAt<3>( in, K, seed, R, in.r[1], v );
+                else if( DM & 16 ) leafAt<4>( in, K, seed, R, in.r[1], v );
+                R[in.r[0]] = v;
+                break;
+            }
+            case fn2::OP_FRACTAL_LEAF:
+            {
+                float v = 0.0f;
+                if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2>( in, K, seed, R, in.r[1] );
+                else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3>( in, K, seed, R, in.r[1] );
+                else if( DM & 16 ) v = fractalLeaf<4>( in, K, seed, R, in.r[1] );
+                R[in.r[0]] = v;
+                break;
+            }
+            case fn2::OP_CELL_VALUE: case fn2::OP_CELL_DISTANCE: case fn2::OP_CELL_LOOKUP:
+                if constexpr( ( FT & fn2::kFeatCell ) != 0 )
+                {
+                    if( ( DM & 4 ) && d == 2 ) cellular<2>( in, K, seed, R );
+                    else if( ( DM & 8 ) && d == 3 ) cellular<3>( in, K, seed, R );
+                    else if( DM & 16 ) cellular<4>( in, K, seed, R );
+                }
+                break;
+            case fn2::OP_WARP_GRADIENT: case fn2::OP_WARP_SIMPLEX: case fn2::OP_WARP_SUPERSIMPLEX:
+                if constexpr( ( FT & fn2::kFeatWarp ) != 0 )
+                {
+                    if( ( DM & 4 ) && d == 2 ) warpOp<2>( in, K, seed, R );
+                    else if( ( DM & 8 ) && d == 3 ) warpOp<3>( in, K, seed, R );
+                    else if( DM & 16 ) warpOp<4>( in, K, seed, R );
+                }
+                break;
+            case fn2::OP_WHITE:
+            if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+            {
+                // White (BasicGenerators.inl:45-59): hash of the raw position bits
+                const int32_t prime[4] = { kPX, kPY, kPZ, kPW };
+                int32_t h = seed;
+                for( int ax = 0; ax < d; ax++ )
+                {
+                    int32_t b = fbits( R[in.r[1] + ax] );
+                    h ^= imul( b ^ ( b >> 16 ), prime[ax] );
+                }
+                R[in.r[0]] = scaleOutput( valueCoordFinal( h ), K[1], K[2], K[3] );
+                break;
+            }
+            case fn2::OP_CHECKERBOARD:
+            if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+            {
+                int32_t v = roundToInt( R[in.r[1]] * K[0] );
+                for( int ax = 1; ax < d; ax++ ) v ^= roundToInt( R[in.r[1] + ax] * K[0] );
+                R[in.r[0]] = scaleOutput( ibits( ishl( v & 1, 30 ) ), K[1], K[2], K[3] );
+                break;
+            }
+            case fn2::OP_SINEWAVE:
+            if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+            {
+                float r = sinf( R[in.r[1] + d - 1] * K[0] );
+                for( int ax = d - 2; ax >= 0; ax-- ) r = sinf( R[in.r[1] + ax] * K[0] ) * r;
+                R[in.r[0]] = scaleOutput( r, K[1], K[2], K[3] );
+                break;
+            }
+            case fn2::OP_CONST: R[in.r[0]] = K[0]; break;
+            case fn2::OP_MOV: R[in.r[0]] = R[in.r[1]]; break;
+            case fn2::OP_ADD: R[in.r[0]] = R[in.r[1]] + R[in.r[2]]; break;
+            case fn2::OP_SUB: R[in.r[0]] = R[in.r[1]] - R[in.r[2]]; break;
+            case fn2::OP_MUL: R[in.r[0]] = R[in.r[1]] * R[in.r[2]]; break;
+            case fn2::OP_DIV: R[in.r[0]] = R[in.r[1]] / R[in.r[2]]; break;
+            case fn2::OP_MIN: R[in.r[0]] = fminf_( R[in.r[1]], R[in.r[2]] ); break;
+            case fn2::OP_MAX: R[in.r[0]] = fmaxf_( R[in.r[1]], R[in.r[2]] ); break;
+            case fn2::OP_FMOD: R[in.r[0]] = __builtin_fmodf( R[in.r[1]], R[in.r[2]] ); break;
+            case fn2::OP_ABS: R[in.r[0]] = fabs_( R[in.r[1]] ); break;
+            case fn2::OP_SCALE:
+                for( int ax = 0; ax < d; ax++ ) R[in.r[0] + ax] = R[in.r[1] + ax] * K[0];
+                break;
+            case fn2::OP_SCALE_AXES:
+                for( int ax = 0; ax < d; ax++ ) R[in.r[0] + ax] = R[in.r[1] + ax] * K[ax];
+                break;
+            case fn2::OP_MUL_VEC:
+            {
+                float s = R[in.r[2]];
+                for( int ax = 0; ax < d; ax++ ) R[in.r[0] + ax] = R[in.r[1] + ax] * s;
+                break;
+            }
+            case fn2::OP_ADD_VEC:
+                for( int ax = 0; ax < d; ax++ ) R[in.r[0] + ax] = R[in.r[1] + ax] + R[in.r[2] + ax];
+                break;
+            case fn2::OP_SUB_VEC:
+                for( int ax = 0; ax < d; ax++ ) R[in.r[0] + ax] = R[in.r[1] + ax] - R[in.r[2] + ax];
+                break;
+            case fn2::OP_MOV_VEC:
+                for( int ax = 0; ax < d; ax++ ) R[in.r[0] + ax] = R[in.r[1] + ax];
+                break;
+            case fn2::OP_FBM_AMP:
+            {
+                float amp = R[in.r[0]] * lerp( 1.0f, ( R[in.r[1]] + 1.0f ) * 0.5f, R[in.r[2]] );
+                R[in.r[0]] = amp * R[in.r[3]];
+                break;
+            }
+            case fn2::OP_RIDGED_AMP:
+            {
+                float amp = R[in.r[0]] * lerp( 1.0f, 1.0f - R[in.r[1]], R[in.r[2]] );
+                R[in.r[0]] = amp * R[in.r[3]];
+                break;
+            }
+            case fn2::OP_WARP_AMP:
+            {
+                float strength = __builtin_sqrtf( R[in.r[1]] );
+                float amp = R[in.r[0]] * lerp( 1.0f, 1.0f - strength, R[in.r[2]] );
+                R[in.r[0]] = amp * R[in.r[3]];
+                break;
+            }
+            case fn2::OP_FBM_SUM: R[in.r[0]] = R[in.r[0]] + R[in.r[1]] * R[in.r[2]]; break;
+            case fn2::OP_RIDGED_TERM:
+            {
+                float t = ( R[in.r[1]] * -2.0f + 1.0f ) * R[in.r[2]];
+                R[in.r[0]] = in.p[0] ? R[in.r[0]] + t : t;
+                break;
+            }
+            case fn2::OP_PINGPONG:
+            {
+                // PingPong (Modifiers.inl:189-200)
+                float t = R[in.r[1]] * R[in.r[2]];
+                t -= ffloor( t * 0.5f ) * 2.0f;
+                R[in.r[0]] = t < 1.0f ? t : 2.0f - t;
+                break;
+            }
+            case fn2::OP_MINSMOOTH:
+            {
+                // MinSmooth / MaxSmooth (Blends.inl:124-162)
+                float x = R[in.r[1]], y = R[in.r[2]];
+                if( in.p[0] ) { x = -x; y = -y; }
+                float s = fmaxf_( 1.17549435e-38f, fabs_( R[in.r[3]] ) );
+                float h = fmaxf_( s - fabs_( x - y ), 0.0f );
+                h *= recip( s );
+                float r = fnmadd( 1.0f / 6.0f, h * h * h * s, fminf_( x, y ) );
+                R[in.r[0]] = in.p[0] ? -r : r;
+                break;
+            }
+            case fn2::OP_FADE:
+            {
+                // Fade (Blends.inl:164-199)
+                float fmin = R[in.r[4]], range = R[in.r[5]] - fmin;
+                float f = ( R[in.r[3]] - fmin ) / range;
+                f = fmaxf_( 0.0f, fminf_( 1.0f, f ) );
+                if( in.p[0] == 1 ) f = interpHermite( f );
+                else if( in.p[0] == 2 ) f = interpQuintic( f );
+                f = range == 0.0f ? 0.5f : f;
+                R[in.r[0]] = lerp( R[in.r[1]], R[in.r[2]], f );
+                break;
+            }
+            case fn2::OP_REMAP:
+            {
+                // Remap (Modifiers.inl:81-99): true division
+                float fromMin = R[in.r[2]], toMin = R[in.r[4]], toMax = R[in.r[5]];
+                float r = toMin + ( ( R[in.r[1]] - fromMin ) / ( R[in.r[3]] - fromMin ) * ( toMax - toMin ) );
+                if( in.p[0] ) { r = fmaxf_( r, toMin ); r = fminf_( r, toMax ); }
+                R[in.r[0]] = r;
+                break;
+            }
+            case fn2::OP_TERRACE:
+            {
+                // Terrace (Modifiers.inl:135-181)
+                float s = R[in.r[1]] * K[0];
+                float rounded = __builtin_rintf( s );
+                bool dynamic = in.r[2] != fn2::kNoReg;
+                float smooth = dynamic ? R[in.r[2]] : 0.0f;
+                bool on = dynamic ? smooth != 0.0f : in.p[1] != 0;
+                if( on )
+                {
+                    float diff = rounded - s;
+                    bool neg = diff < 0.0f;
+                    diff = 0.5f - fabs_( diff );
+                    diff *= dynamic ? 1.0f + recip( smooth ) : K[2];
+                    diff = fminf_( diff, 0.5f );
+                    diff = neg ? 0.5f - diff : diff - 0.5f;
+                    rounded += diff;
+                }
+                R[in.r[0]] = rounded * K[1];
+                break;
+            }
+            case fn2::OP_POWF:
+                R[in.r[0]] = powf( fmaxf_( fabs_( R[in.r[1]] ), 1.17549435e-38f ), R[in.r[2]] );
+                break;
+            case fn2::OP_POWI:
+            {
+                float v = R[in.r[1]], r = v * v;
+                for( int i = 2; i < (int)in.p[0]; i++ ) r *= v;
+                R[in.r[0]] = r;
+                break;
+            }
+            case fn2::OP_SSQRT:
+            {
+                float v = R[in.r[1]];
+                R[in.r[0]] = ibits( fbits( __builtin_sqrtf( fabs_( v ) ) ) | ( fbits( v ) & (int32_t)0x80000000u ) );
+                break;
+            }
+            case fn2::OP_RGBA8:
+            {
+                float s = fminf_( R[in.r[1]], K[1] );
+                s = fmaxf_( s, K[0] );
+                s -= K[0];
+                s *= K[2];
+                int32_t b = roundToInt( s );
+                R[in.r[0]] = ibits( (int32_t)( 255u << 24 ) | b | ishl( b, 8 ) | ishl( b, 16 ) );
+                break;
+            }
+            case fn2::OP_ROTATE2:
+            {
+                // DomainRotate 2D yaw-only branch (Modifiers.inl:38-43)
+                float x = R[in.r[1]], y = R[in.r[1] + 1];
+                R[in.r[0]] = fnmadd( y, K[1], x * K[0] );
+                R[in.r[0] + 1] = fmadd( x, K[1], y * K[0] );
+                break;
+            }
+            case fn2::OP_ROTATE3:
+            {
+                // DomainRotate 3D matrix (Modifiers.inl:48-54)
+                float x = R[in.r[1]], y = R[in.r[1] + 1], z = R[in.r[1] + 2];
+                R[in.r[0]] = fmadd( x, K[0], fmadd( y, K[1], z * K[2] ) );
+                R[in.r[0] + 1] = fmadd( x, K[3], fmadd( y, K[4], z * K[5] ) );
+                R[in.r[0] + 2] = fmadd( x, K[6], fmadd( y, K[7], z * K[8] ) );
+                break;
+            }
+            case fn2::OP_ROTATE_PLANE:
+            {
+                // DomainRotatePlane (Modifiers.inl:354-388), plain multiplies and adds
+                float x = R[in.r[1]], y = R[in.r[1] + 1], z = R[in.r[1] + 2];
+                float nx = x, ny = y, nz = z;
+                if( in.p[0] == 0 )
+                {
+                    float xy = x + y, s2 = xy * -0.211324865405187f;
+                    nz = z * 0.577350269189626f;
+                    nx = x + s2 - nz;
+                    ny = y + s2 - nz;
+                    nz = nz + xy * 0.577350269189626f;
+                }
+                else if( in.p[0] == 1 )
+                {
+                    float xz = x + z, s2 = xz * -0.211324865405187f;
+                    ny = y * 0.577350269189626f;
+                    nx = x + s2 - ny;
+                    nz = z + s2 - ny;
+                    ny = ny + xz * 0.577350269189626f;
+                }
+                R[in.r[0]] = nx; R[in.r[0] + 1] = ny; R[in.r[0] + 2] = nz;
+                break;
+            }
+            case fn2::OP_GRADIENT_AXIS:
+                R[in.r[0]] = fmadd( R[in.r[1]] + R[in.r[2]], K[0], R[in.r[0]] );
+                break;
+            case fn2::OP_DISTANCE:
+                if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+                {
+                    if( ( DM & 4 ) && d == 2 ) distanceOp<2>( in, R );
+                    else if( ( DM & 8 ) && d == 3 ) distanceOp<3>( in, R );
+                    else if( DM & 16 ) distanceOp<4>( in, R );
+                }
+                break;
+            default:
+                break;
+            }
+        }
+        return R[a.resultReg];
+    }
+
+    FN2D void startPosition( const fn2::GenArgs& a, uint32_t idx, float* R )
+    {
+        if( a.kind == fn2::SRC_POSITIONS )
+        {
+            for( int ax = 0; ax < a.dims; ax++ ) R[ax] = a.offset[ax] + a.pos[ax][idx];
+        }
+        else if( a.kind == fn2::SRC_TILEABLE )
+        {
+            uint32_t yi = fastDiv( a.div[0], idx );
+            uint32_t xi = idx - yi * a.div[0].d;
+            // per-column and per-row torus coordinates, tabulated by the host
+            R[0] = a.pos[0][xi];
+            R[1] = a.pos[1][yi];
+            R[2] = a.pos[2][xi];
+            R[3] = a.pos[3][yi];
+        }
+        else
+        {
+            uint32_t rem = idx;
+            for( int ax = 0; ax < a.dims - 1; ax++ )
+            {
+                uint32_t q = fastDiv( a.div[ax], rem );
+                R[ax] = fmadd( (float)(int32_t)( rem - q * a.div[ax].d ), a.step[ax], a.offset[ax] );
+                rem = q;
+            }
+            R[a.dims - 1] = fmadd( (float)(int32_t)rem, a.step[a.dims - 1], a.offset[a.dims - 1] );
+        }
+    }
+}

@@ -1,0 +1,69 @@
+"""The benchmark workloads of the generator path, as node trees and inputs.
+
+Frequencies are given the way FastNoise2's node editor shows them: a node
+with frequency f has Feature Scale 1/f (BasicGenerators.h:20-24 of the
+reference: mFrequency = 1 / scale), so 0.02 -> 50, 0.005 -> 200, 0.01 -> 100.
+"""
+import numpy as np
+
+from .tree import NodeSpec
+
+S = NodeSpec
+
+
+def perlin_3d():
+    """Perlin, frequency 0.02: GenUniformGrid3D 64^3 at (0,0,0), seed 1337."""
+    return S("Perlin", feature_scale=50)
+
+
+def fbm_simplex_2d():
+    """FractalFBm(Simplex), 5 octaves, gain 0.5, lacunarity 2, weighted strength 0,
+    frequency 0.005: GenUniformGrid2D 4096^2 at (-2048,-2048) seed 42, and
+    GenTileable2D 1024^2."""
+    return S("FractalFBm", source=S("Simplex", feature_scale=200), octaves=5, gain=0.5,
+             lacunarity=2.0, weighted_strength=0.0)
+
+
+def warped_cellular_3d():
+    """DomainWarpGradient (amplitude 30, frequency 0.01) over CellularDistance
+    (Euclidean, Index0Add1, jitter 1, frequency 0.02): GenUniformGrid3D 256^3
+    at (0,0,0) seed 1337, and GenPositionArray3D on 16,777,216 positions."""
+    cell = S("CellularDistance", feature_scale=50, distance_function="Euclidean",
+             return_type="Index0Add1", distance_index_0=0, distance_index_1=1, grid_jitter=1.0)
+    return S("DomainWarpGradient", source=cell, feature_scale=100, warp_amplitude=30.0)
+
+
+FBM_SUPERSIMPLEX_SUM = 1.96875  # amplitude sum of 6 octaves at gain 0.5 (FBm output is not normalised)
+
+
+def serving_tree():
+    """Add( Remap( DomainWarpFractalProgressive( 3 octaves, DomainWarpSimplex(
+    amplitude 40, FractalFBm( SuperSimplex, 6 octaves ) ) ), from +-1.96875 to
+    [0,1] ), Multiply( CellularValue, 0.3 ) )."""
+    fbm = S("FractalFBm", source=S("SuperSimplex"), octaves=6)
+    warp = S("DomainWarpSimplex", source=fbm, warp_amplitude=40.0)
+    prog = S("DomainWarpFractalProgressive", domain_warp_source=warp, octaves=3)
+    remap = S("Remap", source=prog, from_min=-FBM_SUPERSIMPLEX_SUM, from_max=FBM_SUPERSIMPLEX_SUM, to_min=0.0, to_max=1.0)
+    cell = S("Multiply", lhs=S("CellularValue"), rhs=0.3)
+    return S("Add", lhs=remap, rhs=cell)
+
+
+def splitmix64_positions(n, seed=7, lo=-1000.0, hi=1000.0):
+    """SplitMix64 stream seeded `seed`; draws 3i, 3i+1, 3i+2 give point i's
+    x, y, z = lo + (hi - lo) * ((r >> 40) * 2^-24), computed in float64 and
+    rounded once to float32."""
+    out = np.empty((3, n), dtype=np.float32)
+    golden = np.uint64(0x9E3779B97F4A7C15)
+    chunk = 1 << 21
+    with np.errstate(over="ignore"):
+        for base in range(0, 3 * n, chunk):
+            k = np.arange(base, min(3 * n, base + chunk), dtype=np.uint64)
+            z = np.uint64(seed) + (k + np.uint64(1)) * golden
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            u = (z >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+            v = (lo + (hi - lo) * u).astype(np.float32)
+            idx = k.astype(np.int64)
+            out[idx % 3, idx // 3] = v
+    return out[0], out[1], out[2]

@@ -1,0 +1,110 @@
+/*
+ * Extensions of the MI355X FastNoise2 backend beyond FastNoise2's C ABI.
+ *
+ * These entry points have no counterpart in the reference's
+ * include/FastNoise/FastNoise_C.h. They exist for callers that keep their
+ * data on the GPU: generation into device buffers on a caller stream,
+ * device memory helpers, timing, program introspection and the
+ * continuous-batching serving engine. Plain C types only.
+ */
+#ifndef FN2_GPU_H
+#define FN2_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "FastNoise/Utility/Export.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Where the points of one generation call come from. */
+enum fn2SourceKind
+{
+    FN2_SOURCE_GRID = 0,        /* GenUniformGrid2D/3D/4D */
+    FN2_SOURCE_POSITIONS = 1,   /* GenPositionArray2D/3D/4D: pos[] are device pointers */
+    FN2_SOURCE_TILEABLE = 2,    /* GenTileable2D (count[0], count[1] = sizes) */
+};
+
+/* One generation call in device form. Grid: count[]/offset[]/step[] per axis.
+ * Positions: count[0] points, pos[axis] device arrays, offset[] added. */
+typedef struct fn2GenSpec
+{
+    int kind;
+    int dims;
+    int count[4];
+    float offset[4];
+    float step[4];
+    const float* pos[4];
+    int seed;
+} fn2GenSpec;
+
+/* Generates into device memory dOut on `stream` (a hipStream_t, NULL = the
+ * library stream). outputMinMax: host float[2] (the call then waits for the
+ * result) or NULL (fully asynchronous). Returns 0 or a nonzero error code;
+ * fn2GetLastError() describes it. */
+FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, float* dOut, float* outputMinMax, void* stream );
+
+/* Times `iters` back-to-back generator launches (each writes all points of
+ * `spec` to dOut and folds them into the call's min/max) after `warmup`
+ * launches, with HIP events on the launch stream. msPerLaunch: average event
+ * time per launch; wallMsPerLaunch: host wall time per launch from the first
+ * launch to the completion of the last. */
+FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, float* dOut, int warmup, int iters, double* msPerLaunch, double* wallMsPerLaunch );
+
+/* Device memory helpers (hipMalloc / hipFree / hipMemcpy, kind: 0 H2D, 1 D2H, 2 D2D). */
+FASTNOISE_API void* fn2Malloc( size_t bytes );
+FASTNOISE_API void fn2Free( void* p );
+FASTNOISE_API int fn2Memcpy( void* dst, const void* src, size_t bytes, int kind );
+FASTNOISE_API int fn2DeviceSynchronize( void );
+FASTNOISE_API int fn2DeviceCount( void );
+
+/* Last error of this thread ("" when none). */
+FASTNOISE_API const char* fn2GetLastError( void );
+
+/* Compiled-program facts for a tree evaluated in `dims` dimensions. */
+FASTNOISE_API int fn2GetProgramInfo( const void* node, int dims, int* instructions, int* registers, int* dimMask, int* featureMask );
+
+/* ------------------------------------------------------------------------
+ * Serving engine: continuous batching of chunk requests.
+ * ------------------------------------------------------------------------ */
+
+/* Batching policies */
+enum fn2ServeMode
+{
+    FN2_SERVE_PLAIN = 0,    /* one launch + copy per request (the per-chunk baseline) */
+    FN2_SERVE_BATCHED = 1,  /* requests packed into one launch per batch, overlapped with host work */
+    FN2_SERVE_GRAPH = 2,    /* batched, each batch replayed through a captured hipGraph */
+};
+
+/* Creates an engine. hbmBudgetBytes bounds the device bytes held by queued and
+ * running requests (outputs + positions); maxBatchPoints bounds the points of
+ * one batch. Returns NULL on failure. */
+FASTNOISE_API void* fn2ServeCreate( size_t hbmBudgetBytes, int maxBatchPoints, int mode );
+FASTNOISE_API void fn2ServeDestroy( void* engine );
+
+/* Registers a node tree (encoded string, FastNoise2 wire format) and returns
+ * its id (>= 0) or -1. Trees stay resident until the engine is destroyed. */
+FASTNOISE_API int fn2ServeAddTree( void* engine, const char* encodedTree );
+
+/* Submits a request: tree id, generation spec (grid or tileable; host
+ * positions are not accepted), host output buffer of the spec's point count.
+ * Returns a request id (>= 0) or -1 when the request can never fit the
+ * budget. The call does not block on the GPU. */
+FASTNOISE_API int64_t fn2ServeSubmit( void* engine, int treeId, const fn2GenSpec* spec, float* hostOut );
+
+/* Waits for a request; fills its output buffer and min/max. latencyMs:
+ * submit-to-completion time. Returns 0 or an error code. */
+FASTNOISE_API int fn2ServeWait( void* engine, int64_t requestId, float* outputMinMax, double* latencyMs );
+
+/* Waits for everything submitted so far. */
+FASTNOISE_API int fn2ServeDrain( void* engine );
+
+/* Engine counters: batches launched, requests completed, peak device bytes. */
+FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* completed, int64_t* peakBytes );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -315,6 +315,7 @@ namespace
     int run( const DriverArgs& a, int mode, int lanes, int threads, float* minmax )
     {
         if( !a.root ) return -1;
+#ifndef FN2O_COUNTING
         if( lanes == 16 )
         {
             __builtin_cpu_init();
@@ -329,6 +330,10 @@ namespace
                 return 0;
             }
         }
+#else
+        (void)lanes;
+        threads = 1;
+#endif
         run_mode<1>( a, mode, threads, minmax );
         return 0;
     }
@@ -499,4 +504,26 @@ int fn2o_base64_decode( const char* in, unsigned char* out, int cap )
     return (int)v.size();
 }
 
+#ifdef FN2O_COUNTING
+// Op counts of the calls made since the last reset (this thread).
+void fn2o_counters( unsigned long long* flop, unsigned long long* valu )
+{
+    *flop = fn2o_count::flop;
+    *valu = fn2o_count::valu;
+}
+
+void fn2o_reset_counters( void )
+{
+    fn2o_count::flop = 0;
+    fn2o_count::valu = 0;
+}
+#endif
+
 } // extern "C"
+
+#ifdef FN2O_COUNTING
+namespace fn2o_count
+{
+    thread_local uint64_t flop = 0, valu = 0;
+}
+#endif

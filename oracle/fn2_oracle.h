@@ -3,7 +3,7 @@
  * generator path.
  *
  * The oracle is a CPU restatement of the reference algorithm (node math in
- * include/FastNoise/Generators/*.inl, drivers in Generator.inl, wire format
+ * the include/FastNoise/Generators .inl files, drivers in Generator.inl, wire format
  * in src/FastNoise/Metadata.cpp and Base64.h of FastNoise2 v1.1.1). It is the
  * checker for the parity tests and the host baseline of bench.py; the product
  * library never links it.
