@@ -1,15 +1,22 @@
 // HIP kernels for gfx950: the node-tree program interpreter and the output
 // range reduction.
 //
-// One lane evaluates one point: it forms the point's position (uniform grid,
-// position arrays or the tileable 4D torus of Generator.inl:79-433), runs the
-// wave-uniform instruction list of the compiled tree on a 32-entry register
-// file held in VGPRs (uniform indices -> GPR indexing, no scratch), writes one
-// float with a coalesced store and folds it into a block min/max that is
-// published with one atomic pair per block.
+// One lane evaluates one point at a time: it forms the point's position
+// (uniform grid, position arrays or the tileable 4D torus of
+// Generator.inl:79-433), runs the wave-uniform instruction list of the
+// compiled tree on a 32-entry register file held in VGPRs (uniform indices ->
+// GPR indexing, no scratch) and writes one float with a coalesced store.
+//
+// Output range: every lane keeps a running min/max in registers; a block
+// reduces them through wave shuffles and LDS and publishes one atomic pair.
+// Same-address atomics serialise in L2, so the number of blocks, not of
+// waves, bounds their cost: the single-call kernel runs a grid-stride loop
+// over a grid sized to fill the GPU (a few thousand blocks), and the batched
+// kernel gives each block a tile of several points per lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "fn2_interp.h"
+#include "fn2_kernel_args.h"
 
 using namespace fn2d;
 using namespace fn2i;
@@ -17,66 +24,96 @@ using namespace fn2i;
 namespace
 {
     template<int BLOCK>
-    __device__ __forceinline__ void reduceMinMax( float mn, float mx, int32_t* keys )
+    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* keys )
     {
-        // wave reduce, then one atomic pair per wave
+        __shared__ float sMin[BLOCK / 64], sMax[BLOCK / 64];
 #pragma unroll
         for( int off = 32; off > 0; off >>= 1 )
         {
             mn = fminf( mn, __shfl_xor( mn, off ) );
             mx = fmaxf( mx, __shfl_xor( mx, off ) );
         }
-        if( ( threadIdx.x & 63 ) == 0 && mn <= mx )
+        const int wave = threadIdx.x >> 6;
+        if( ( threadIdx.x & 63 ) == 0 )
         {
-            atomicMin( &keys[0], orderedKey( mn ) );
-            atomicMax( &keys[1], orderedKey( mx ) );
+            sMin[wave] = mn;
+            sMax[wave] = mx;
+        }
+        __syncthreads();
+        if( threadIdx.x == 0 )
+        {
+#pragma unroll
+            for( int w = 1; w < BLOCK / 64; w++ )
+            {
+                mn = fminf( mn, sMin[w] );
+                mx = fmaxf( mx, sMax[w] );
+            }
+            if( mn <= mx )
+            {
+                atomicMin( &keys[0], orderedKey( mn ) );
+                atomicMax( &keys[1], orderedKey( mx ) );
+            }
         }
     }
 }
 
+// Single call: grid-stride loop over all points of one request.
 template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate( fn2::GenArgs a )
 {
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    const bool live = i < (uint32_t)a.total;
-    const uint32_t idx = live ? i : (uint32_t)a.total - 1;
     float R[fn2::kMaxRegs];
 #pragma unroll
     for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
-    startPosition( a, idx, R );
-    float v = runProgram<DM, FT>( a, R );
-    if( live ) a.out[i] = v;
-    if( a.minmax )
+    float mn = INFINITY, mx = -INFINITY;
+    const uint32_t total = (uint32_t)a.total;
+    const uint32_t stride = gridDim.x * BLOCK;
+    for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
     {
-        bool ok = live && v == v;
-        reduceMinMax<BLOCK>( ok ? v : INFINITY, ok ? v : -INFINITY, a.minmax );
+        startPosition( a, i, R );
+        const float v = runProgram<DM, FT>( a, R );
+        a.out[i] = v;
+        if( v == v )
+        {
+            mn = fminf( mn, v );
+            mx = fmaxf( mx, v );
+        }
     }
+    if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
 }
 
-// Batched form for the serving layer: every block looks up its job in a
+// Batched form for the serving layer: every block looks up its request in a
 // per-block table, so requests with different trees, sizes and sources run in
-// one launch. `jobs` holds one GenArgs per request; `blockJob[b]` and
-// `blockBase[b]` map block b to (request, first point).
-template<int BLOCK, int DM, int FT>
+// one launch. `jobs` holds one GenArgs per request; block b covers points
+// [blockBase[b], blockBase[b] + BLOCK * PPT) of request blockJob[b] (clipped
+// to the request's size).
+template<int BLOCK, int PPT, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ blockJob, const uint32_t* __restrict__ blockBase )
 {
     // read the job through a pointer (uniform address -> scalar loads) rather
     // than copying it into a private struct
     const fn2::GenArgs& a = jobs[blockJob[blockIdx.x]];
-    const uint32_t i = blockBase[blockIdx.x] + threadIdx.x;
-    const bool live = i < (uint32_t)a.total;
-    const uint32_t idx = live ? i : (uint32_t)a.total - 1;
     float R[fn2::kMaxRegs];
 #pragma unroll
     for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
-    startPosition( a, idx, R );
-    float v = runProgram<DM, FT>( a, R );
-    if( live ) a.out[i] = v;
-    if( a.minmax )
+    float mn = INFINITY, mx = -INFINITY;
+    const uint32_t total = (uint32_t)a.total;
+    const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
+    for( int p = 0; p < PPT; p++ )
     {
-        bool ok = live && v == v;
-        reduceMinMax<BLOCK>( ok ? v : INFINITY, ok ? v : -INFINITY, a.minmax );
+        const uint32_t i = base + p * BLOCK;
+        if( i < total )
+        {
+            startPosition( a, i, R );
+            const float v = runProgram<DM, FT>( a, R );
+            a.out[i] = v;
+            if( v == v )
+            {
+                mn = fminf( mn, v );
+                mx = fmaxf( mx, v );
+            }
+        }
     }
+    if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
 }
 
 __global__ void fn2_minmax_init( int32_t* keys, int n )
@@ -86,6 +123,32 @@ __global__ void fn2_minmax_init( int32_t* keys, int n )
     {
         keys[2 * i] = 0x7fffffff;          // key of +NaN region: above every float
         keys[2 * i + 1] = (int32_t)0x80000000u;
+    }
+}
+
+namespace
+{
+    constexpr int kBlock = 256;
+    constexpr int kBatchPPT = 4;
+
+    // Blocks of the single-call kernel: enough to fill every CU several times
+    // over, few enough that the per-block range atomics stay negligible.
+    unsigned GridFor( int64_t total )
+    {
+        static int cus = 0;
+        if( cus == 0 )
+        {
+            int dev = 0, n = 0;
+            if( hipGetDevice( &dev ) != hipSuccess || hipDeviceGetAttribute( &n, hipDeviceAttributeMultiprocessorCount, dev ) != hipSuccess || n <= 0 )
+            {
+                (void)hipGetLastError();
+                n = 256;
+            }
+            cus = n;
+        }
+        int64_t need = ( total + kBlock - 1 ) / kBlock;
+        int64_t cap = (int64_t)cus * 16;
+        return (unsigned)( need < cap ? need : cap );
     }
 }
 
@@ -99,15 +162,21 @@ extern "C" {
     X( 4, 7 ) X( 8, 7 ) X( 16, 7 ) \
     X( 28, 7 )
 
+// points per block of the batched kernel (the serving layer places requests
+// on multiples of this)
+int fn2_batch_tile_points( void )
+{
+    return kBlock * kBatchPPT;
+}
+
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream )
 {
     if( args->total <= 0 ) return 0;
-    const int block = 256;
-    unsigned grid = (unsigned)( ( (int64_t)args->total + block - 1 ) / block );
+    unsigned grid = GridFor( args->total );
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        hipLaunchKernelGGL( ( fn2_generate<256, DMV, FTV> ), dim3( grid ), dim3( block ), 0, stream, *args ); \
+        hipLaunchKernelGGL( ( fn2_generate<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), 0, stream, *args ); \
         return (int)hipGetLastError(); \
     }
     FN2_VARIANTS( FN2_TRY )
@@ -121,7 +190,7 @@ int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJ
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        hipLaunchKernelGGL( ( fn2_generate_batch<256, DMV, FTV> ), dim3( nBlocks ), dim3( 256 ), 0, stream, dJobs, dBlockJob, dBlockBase ); \
+        hipLaunchKernelGGL( ( fn2_generate_batch<kBlock, kBatchPPT, DMV, FTV> ), dim3( nBlocks ), dim3( kBlock ), 0, stream, dJobs, dBlockJob, dBlockBase ); \
         return (int)hipGetLastError(); \
     }
     FN2_VARIANTS( FN2_TRY )

@@ -1,12 +1,583 @@
-// Serving engine (continuous batching): under construction.
+// Serving engine: continuous batching of chunk-generation requests.
+//
+// Requests (tree, grid or tileable spec, host output buffer) are queued by
+// fn2ServeSubmit from any thread. A dispatcher thread admits queued requests
+// FIFO into the next free batch slot as long as the slot's point capacity
+// allows, writes one job record per request (GenArgs pointing at the
+// request's output range in the slot's device arena and at its own min/max
+// key pair) plus a block -> (job, first point) table, and launches the batch
+// kernel once per kernel variant present in the batch, followed by the copy
+// of the slot's outputs and keys into pinned host memory. Slots run on their
+// own streams, so a batch formed from requests that arrived while the
+// previous one was on the GPU overlaps with it. A completion thread retires
+// slots in launch order: it copies each request's values into the caller's
+// buffer, decodes its range and wakes the waiters.
+//
+// Device memory: slots * (capacity * 4 bytes + tables) is allocated once at
+// creation and must fit the HBM budget given there; requests wait in the
+// host queue until a slot frees (admission). Trees are compiled once per
+// dimensionality and stay resident.
+//
+// FN2_SERVE_PLAIN runs the same requests one launch + copy + synchronise at a
+// time (the per-chunk API path) for comparison. FN2_SERVE_GRAPH records each
+// batch's table upload, launches and copies into a HIP graph and replays the
+// graph when the next batch has the same shape (same job count, block count
+// and variant groups); job records and tables change, the topology does not.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
 #include "fn2_backend.h"
+#include "fn2_internal.h"
 
 extern "C" {
-FASTNOISE_API void* fn2ServeCreate( size_t, int, int ) { fn2::SetError( "serving engine not built" ); return nullptr; }
-FASTNOISE_API void fn2ServeDestroy( void* ) {}
-FASTNOISE_API int fn2ServeAddTree( void*, const char* ) { return -1; }
-FASTNOISE_API int64_t fn2ServeSubmit( void*, int, const fn2GenSpec*, float* ) { return -1; }
-FASTNOISE_API int fn2ServeWait( void*, int64_t, float*, double* ) { return 1; }
-FASTNOISE_API int fn2ServeDrain( void* ) { return 1; }
-FASTNOISE_API int fn2ServeStats( void*, int64_t*, int64_t*, int64_t* ) { return 1; }
+int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
+int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask, hipStream_t stream );
+int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
+int fn2_batch_tile_points( void );
+}
+
+namespace
+{
+    using Clock = std::chrono::steady_clock;
+    using fn2::GenArgs;
+
+    // points per block of the batched kernel; requests start on multiples of it
+    const int kTile = fn2_batch_tile_points();
+    constexpr int kSlots = 3;
+    constexpr int kMaxJobs = 4096;
+
+    struct Tree
+    {
+        FastNoise::SmartNode<> node;
+        const fn2::DeviceProgram* prog[5] = {};
+        std::string compileError[5];
+    };
+
+    struct Request
+    {
+        int64_t id = 0;
+        int tree = 0;
+        fn2GenSpec spec{};
+        float* hostOut = nullptr;
+        int64_t points = 0;
+        const fn2::DeviceProgram* prog = nullptr;
+        GenArgs args{};
+        Clock::time_point tSubmit, tDone;
+        // placement in its batch slot
+        size_t outOffset = 0;
+        int job = 0;
+        // results
+        bool done = false;
+        int status = 0;
+        float mm[2] = { 0, 0 };
+    };
+
+    struct Slot
+    {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        float* dOut = nullptr;          // capacity floats
+        float* hOut = nullptr;          // pinned
+        int32_t* dKeys = nullptr;       // 2 per job
+        int32_t* hKeys = nullptr;       // pinned
+        GenArgs* dJobs = nullptr;
+        int32_t* dBlockJob = nullptr;
+        uint32_t* dBlockBase = nullptr;
+        uint8_t* hTables = nullptr;     // pinned: jobs | blockJob | blockBase
+        size_t tableBytes = 0;
+        std::vector<Request*> reqs;
+        int64_t points = 0;
+        bool busy = false;
+        // graph mode
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<std::pair<int, int>> graphShape;   // (variant key, blocks) per launch
+        int graphJobs = -1;
+    };
+
+    size_t AlignUp( size_t v, size_t a ) { return ( v + a - 1 ) / a * a; }
+
+    class Engine
+    {
+    public:
+        int mode;
+        int64_t capacity;       // points per batch
+        size_t budget;
+        std::string initError;
+
+        Engine( size_t hbmBudget, int maxBatchPoints, int m ) : mode( m ), capacity( maxBatchPoints ), budget( hbmBudget )
+        {
+        }
+
+        bool Init()
+        {
+            if( capacity < kTile ) capacity = kTile;
+            capacity = (int64_t)AlignUp( (size_t)capacity, kTile );
+            maxBlocks = (int)( capacity / kTile ) + kMaxJobs;
+            size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
+            size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
+            size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
+            size_t perSlot = (size_t)capacity * 4 + jobsBytes + bjBytes + bbBytes + 2 * sizeof( int32_t ) * kMaxJobs;
+            int nSlots = mode == FN2_SERVE_PLAIN ? 1 : kSlots;
+            if( perSlot * nSlots > budget )
+            {
+                initError = "HBM budget too small for the batch capacity (needs " + std::to_string( perSlot * nSlots ) + " bytes)";
+                return false;
+            }
+            slots.resize( nSlots );
+            for( Slot& s : slots )
+            {
+                s.tableBytes = jobsBytes + bjBytes + bbBytes;
+                if( !fn2::HipOk( hipStreamCreateWithFlags( &s.stream, hipStreamNonBlocking ), "serve stream" ) ||
+                    !fn2::HipOk( hipEventCreateWithFlags( &s.done, hipEventDisableTiming ), "serve event" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&s.dOut, (size_t)capacity * 4 ), "serve arena" ) ||
+                    !fn2::HipOk( hipHostMalloc( (void**)&s.hOut, (size_t)capacity * 4, hipHostMallocDefault ), "serve staging" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&s.dKeys, 2 * sizeof( int32_t ) * kMaxJobs ), "serve keys" ) ||
+                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, 2 * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&s.dJobs, s.tableBytes ), "serve tables" ) ||
+                    !fn2::HipOk( hipHostMalloc( (void**)&s.hTables, s.tableBytes, hipHostMallocDefault ), "serve tables" ) )
+                {
+                    initError = fn2::LastError();
+                    return false;
+                }
+                s.dBlockJob = (int32_t*)( (uint8_t*)s.dJobs + jobsBytes );
+                s.dBlockBase = (uint32_t*)( (uint8_t*)s.dJobs + jobsBytes + bjBytes );
+            }
+            deviceBytes = (int64_t)( perSlot * nSlots );
+            dispatcher = std::thread( [this] { DispatchLoop(); } );
+            completer = std::thread( [this] { CompleteLoop(); } );
+            return true;
+        }
+
+        ~Engine()
+        {
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                stop = true;
+            }
+            cvWork.notify_all();
+            cvDone.notify_all();
+            cvSlot.notify_all();
+            if( dispatcher.joinable() ) dispatcher.join();
+            if( completer.joinable() ) completer.join();
+            for( Slot& s : slots )
+            {
+                if( s.stream ) (void)hipStreamSynchronize( s.stream );
+                if( s.exec ) (void)hipGraphExecDestroy( s.exec );
+                if( s.graph ) (void)hipGraphDestroy( s.graph );
+                if( s.dOut ) (void)hipFree( s.dOut );
+                if( s.hOut ) (void)hipHostFree( s.hOut );
+                if( s.dKeys ) (void)hipFree( s.dKeys );
+                if( s.hKeys ) (void)hipHostFree( s.hKeys );
+                if( s.dJobs ) (void)hipFree( s.dJobs );
+                if( s.hTables ) (void)hipHostFree( s.hTables );
+                if( s.done ) (void)hipEventDestroy( s.done );
+                if( s.stream ) (void)hipStreamDestroy( s.stream );
+            }
+            for( auto& kv : requests ) delete kv.second;
+        }
+
+        int AddTree( const char* encoded )
+        {
+            FastNoise::SmartNode<> node = FastNoise::NewFromEncodedNodeTree( encoded );
+            if( !node )
+            {
+                fn2::SetError( "encoded node tree does not decode" );
+                return -1;
+            }
+            std::lock_guard<std::mutex> lk( mu );
+            trees.emplace_back( new Tree );
+            trees.back()->node = node;
+            return (int)trees.size() - 1;
+        }
+
+        int64_t Submit( int treeId, const fn2GenSpec& spec, float* hostOut )
+        {
+            if( !hostOut )
+            {
+                fn2::SetError( "null output buffer" );
+                return -1;
+            }
+            if( spec.kind == FN2_SOURCE_POSITIONS )
+            {
+                fn2::SetError( "the serving engine takes grid and tileable requests" );
+                return -1;
+            }
+            std::unique_ptr<Request> r( new Request );
+            r->tree = treeId;
+            r->spec = spec;
+            r->hostOut = hostOut;
+            std::string err;
+            if( !fn2::FillSpecArgs( spec, r->args, r->points, err ) )
+            {
+                fn2::SetError( err );
+                return -1;
+            }
+            if( r->points > capacity )
+            {
+                fn2::SetError( "request larger than the engine's batch capacity" );
+                return -1;
+            }
+            const fn2::DeviceProgram* prog = nullptr;
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                if( treeId < 0 || treeId >= (int)trees.size() )
+                {
+                    fn2::SetError( "unknown tree id" );
+                    return -1;
+                }
+                Tree& t = *trees[treeId];
+                int d = r->args.dims;
+                if( !t.prog[d] && t.compileError[d].empty() )
+                {
+                    if( !fn2::CompileForDevice( t.node.get(), d, t.prog[d], nullptr ) ) t.compileError[d] = fn2::LastError();
+                }
+                if( !t.prog[d] )
+                {
+                    fn2::SetError( t.compileError[d] );
+                    return -1;
+                }
+                prog = t.prog[d];
+            }
+            r->prog = prog;
+            r->args.ins = prog->dIns;
+            r->args.konst = prog->dKonst;
+            r->args.nIns = (int32_t)prog->ins.size();
+            r->args.resultReg = prog->resultReg;
+            r->tSubmit = Clock::now();
+            Request* raw = r.release();
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                raw->id = nextId++;
+                requests[raw->id] = raw;
+                queue.push_back( raw );
+            }
+            cvWork.notify_one();
+            return raw->id;
+        }
+
+        int Wait( int64_t id, float* mm, double* latencyMs )
+        {
+            std::unique_lock<std::mutex> lk( mu );
+            auto it = requests.find( id );
+            if( it == requests.end() )
+            {
+                fn2::SetError( "unknown request id" );
+                return 1;
+            }
+            Request* r = it->second;
+            cvDone.wait( lk, [&] { return r->done || stop; } );
+            if( !r->done )
+            {
+                fn2::SetError( "engine stopped" );
+                return 1;
+            }
+            if( mm ) { mm[0] = r->mm[0]; mm[1] = r->mm[1]; }
+            if( latencyMs ) *latencyMs = std::chrono::duration<double, std::milli>( r->tDone - r->tSubmit ).count();
+            int status = r->status;
+            requests.erase( it );
+            delete r;
+            if( status ) fn2::SetError( "request failed on the device" );
+            return status;
+        }
+
+        int Drain()
+        {
+            std::unique_lock<std::mutex> lk( mu );
+            cvDone.wait( lk, [&] { return ( queue.empty() && inflight == 0 ) || stop; } );
+            return 0;
+        }
+
+        void Stats( int64_t* b, int64_t* c, int64_t* p )
+        {
+            std::lock_guard<std::mutex> lk( mu );
+            if( b ) *b = batches;
+            if( c ) *c = completed;
+            if( p ) *p = deviceBytes;
+        }
+
+    private:
+        std::mutex mu;
+        std::condition_variable cvWork, cvDone, cvSlot;
+        bool stop = false;
+        std::deque<Request*> queue;
+        std::unordered_map<int64_t, Request*> requests;
+        std::vector<std::unique_ptr<Tree>> trees;
+        std::vector<Slot> slots;
+        std::deque<int> launched;      // slot indices in launch order
+        int inflight = 0;
+        int64_t nextId = 0;
+        int64_t batches = 0;
+        int64_t completed = 0;
+        int64_t deviceBytes = 0;
+        int maxBlocks = 0;
+        int nextSlot = 0;
+        std::thread dispatcher, completer;
+
+        // Fills slot s with queued requests (FIFO) up to its capacity; the plain
+        // mode takes one request per launch. Called with mu held.
+        void Gather( Slot& s )
+        {
+            s.reqs.clear();
+            s.points = 0;
+            while( !queue.empty() && (int)s.reqs.size() < kMaxJobs )
+            {
+                if( mode == FN2_SERVE_PLAIN && !s.reqs.empty() ) break;
+                Request* r = queue.front();
+                // block-aligned placement keeps every block inside one request
+                int64_t span = ( r->points + kTile - 1 ) / kTile * kTile;
+                if( s.points + span > capacity ) break;
+                queue.pop_front();
+                r->outOffset = (size_t)s.points;
+                s.points += span;
+                s.reqs.push_back( r );
+            }
+        }
+
+        static int VariantKey( const fn2::DeviceProgram* p ) { return p->dimMask * 16 + p->featMask; }
+
+        bool Launch( Slot& s )
+        {
+            // jobs sorted by kernel variant so that each variant's blocks are contiguous
+            std::vector<Request*> order( s.reqs );
+            std::stable_sort( order.begin(), order.end(), []( Request* a, Request* b ) { return VariantKey( a->prog ) < VariantKey( b->prog ); } );
+            GenArgs* hJobs = (GenArgs*)s.hTables;
+            size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
+            size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
+            int32_t* hBJ = (int32_t*)( s.hTables + jobsBytes );
+            uint32_t* hBB = (uint32_t*)( s.hTables + jobsBytes + bjBytes );
+            std::vector<std::pair<int, int>> shape;   // (variant key, blocks)
+            int nb = 0;
+            for( size_t j = 0; j < order.size(); j++ )
+            {
+                Request* r = order[j];
+                r->job = (int)j;
+                GenArgs a = r->args;
+                a.out = s.dOut + r->outOffset;
+                a.minmax = s.dKeys + 2 * j;
+                hJobs[j] = a;
+                int rb = (int)( ( r->points + kTile - 1 ) / kTile );
+                for( int b = 0; b < rb; b++ )
+                {
+                    hBJ[nb + b] = (int32_t)j;
+                    hBB[nb + b] = (uint32_t)b * kTile;
+                }
+                int key = VariantKey( r->prog );
+                if( shape.empty() || shape.back().first != key ) shape.emplace_back( key, rb );
+                else shape.back().second += rb;
+                nb += rb;
+            }
+            const int nJobs = (int)order.size();
+            // the blocks of the tables that are used: copy jobs, then block tables
+            if( mode == FN2_SERVE_GRAPH )
+            {
+                if( s.exec && ( s.graphJobs != nJobs || s.graphShape != shape ) )
+                {
+                    (void)hipGraphExecDestroy( s.exec );
+                    (void)hipGraphDestroy( s.graph );
+                    s.exec = nullptr;
+                    s.graph = nullptr;
+                }
+                if( !s.exec )
+                {
+                    if( !fn2::HipOk( hipStreamBeginCapture( s.stream, hipStreamCaptureModeThreadLocal ), "begin capture" ) ) return false;
+                    bool ok = Enqueue( s, shape, nJobs, nb );
+                    hipGraph_t g = nullptr;
+                    hipError_t e = hipStreamEndCapture( s.stream, &g );
+                    if( !ok || !fn2::HipOk( e, "end capture" ) ) return false;
+                    s.graph = g;
+                    if( !fn2::HipOk( hipGraphInstantiate( &s.exec, s.graph, nullptr, nullptr, 0 ), "graph instantiate" ) ) return false;
+                    s.graphJobs = nJobs;
+                    s.graphShape = shape;
+                }
+                if( !fn2::HipOk( hipGraphLaunch( s.exec, s.stream ), "graph launch" ) ) return false;
+            }
+            else if( !Enqueue( s, shape, nJobs, nb ) ) return false;
+            if( !fn2::HipOk( hipEventRecord( s.done, s.stream ), "serve event" ) ) return false;
+            s.reqs.swap( order );
+            return true;
+        }
+
+        // Stream work of one batch: table upload, range-key reset, one batch
+        // launch per kernel variant, copies of outputs and keys to pinned memory.
+        bool Enqueue( Slot& s, const std::vector<std::pair<int, int>>& shape, int nJobs, int nb )
+        {
+            size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
+            size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
+            size_t usedJob = sizeof( GenArgs ) * nJobs;
+            if( !fn2::HipOk( hipMemcpyAsync( s.dJobs, s.hTables, usedJob, hipMemcpyHostToDevice, s.stream ), "upload jobs" ) ) return false;
+            if( !fn2::HipOk( hipMemcpyAsync( s.dBlockJob, s.hTables + jobsBytes, sizeof( int32_t ) * nb, hipMemcpyHostToDevice, s.stream ), "upload blocks" ) ) return false;
+            if( !fn2::HipOk( hipMemcpyAsync( s.dBlockBase, s.hTables + jobsBytes + bjBytes, sizeof( uint32_t ) * nb, hipMemcpyHostToDevice, s.stream ), "upload blocks" ) ) return false;
+            if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, s.stream ), "range reset" ) ) return false;
+            int first = 0;
+            for( auto& g : shape )
+            {
+                int dm = g.first / 16, fm = g.first % 16;
+                if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, s.dBlockJob + first, s.dBlockBase + first, g.second, dm, fm, s.stream ), "batch launch" ) ) return false;
+                first += g.second;
+            }
+            // every request occupies whole blocks from offset 0: nb blocks cover them all
+            size_t outBytes = (size_t)nb * kTile * 4;
+            if( !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, outBytes, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
+            if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, 2 * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
+            return true;
+        }
+
+        void DispatchLoop()
+        {
+            for( ;; )
+            {
+                int si;
+                {
+                    std::unique_lock<std::mutex> lk( mu );
+                    cvWork.wait( lk, [&] { return stop || ( !queue.empty() && !slots[nextSlot].busy ); } );
+                    if( stop ) return;
+                    si = nextSlot;
+                    Gather( slots[si] );
+                    slots[si].busy = true;
+                    inflight++;
+                    nextSlot = ( nextSlot + 1 ) % (int)slots.size();
+                }
+                Slot& s = slots[si];
+                bool ok = Launch( s );
+                {
+                    std::lock_guard<std::mutex> lk( mu );
+                    if( !ok )
+                    {
+                        for( Request* r : s.reqs ) r->status = 1;
+                    }
+                    launched.push_back( ok ? si : -1 - si );
+                    batches++;
+                }
+                cvSlot.notify_all();
+            }
+        }
+
+        void CompleteLoop()
+        {
+            for( ;; )
+            {
+                int code;
+                {
+                    std::unique_lock<std::mutex> lk( mu );
+                    cvSlot.wait( lk, [&] { return stop || !launched.empty(); } );
+                    if( launched.empty() ) return;
+                    code = launched.front();
+                    launched.pop_front();
+                }
+                int si = code >= 0 ? code : -1 - code;
+                Slot& s = slots[si];
+                bool ok = code >= 0 && fn2::HipOk( hipEventSynchronize( s.done ), "serve batch" );
+                if( ok )
+                {
+                    for( Request* r : s.reqs )
+                    {
+                        memcpy( r->hostOut, s.hOut + r->outOffset, (size_t)r->points * 4 );
+                        int32_t k0 = s.hKeys[2 * r->job], k1 = s.hKeys[2 * r->job + 1];
+                        if( k0 == 0x7fffffff && k1 == (int32_t)0x80000000u )
+                        {
+                            r->mm[0] = __builtin_inff();
+                            r->mm[1] = -__builtin_inff();
+                        }
+                        else
+                        {
+                            r->mm[0] = fn2::DecodeKey( k0 );
+                            r->mm[1] = fn2::DecodeKey( k1 );
+                        }
+                    }
+                }
+                auto now = Clock::now();
+                {
+                    std::lock_guard<std::mutex> lk( mu );
+                    for( Request* r : s.reqs )
+                    {
+                        if( !ok ) r->status = 1;
+                        r->tDone = now;
+                        r->done = true;
+                        completed++;
+                    }
+                    s.reqs.clear();
+                    s.busy = false;
+                    inflight--;
+                }
+                cvDone.notify_all();
+                cvWork.notify_all();
+            }
+        }
+    };
+}
+
+extern "C" {
+
+FASTNOISE_API void* fn2ServeCreate( size_t hbmBudgetBytes, int maxBatchPoints, int mode )
+{
+    fn2::ClearError();
+    if( !fn2::EnsureDevice() ) return nullptr;
+    if( mode < FN2_SERVE_PLAIN || mode > FN2_SERVE_GRAPH || maxBatchPoints <= 0 )
+    {
+        fn2::SetError( "bad serving engine arguments" );
+        return nullptr;
+    }
+    Engine* e = new Engine( hbmBudgetBytes, maxBatchPoints, mode );
+    if( !e->Init() )
+    {
+        fn2::SetError( e->initError );
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+FASTNOISE_API void fn2ServeDestroy( void* engine )
+{
+    delete static_cast<Engine*>( engine );
+}
+
+FASTNOISE_API int fn2ServeAddTree( void* engine, const char* encodedTree )
+{
+    fn2::ClearError();
+    if( !engine || !encodedTree ) { fn2::SetError( "null argument" ); return -1; }
+    return static_cast<Engine*>( engine )->AddTree( encodedTree );
+}
+
+FASTNOISE_API int64_t fn2ServeSubmit( void* engine, int treeId, const fn2GenSpec* spec, float* hostOut )
+{
+    fn2::ClearError();
+    if( !engine || !spec ) { fn2::SetError( "null argument" ); return -1; }
+    return static_cast<Engine*>( engine )->Submit( treeId, *spec, hostOut );
+}
+
+FASTNOISE_API int fn2ServeWait( void* engine, int64_t requestId, float* outputMinMax, double* latencyMs )
+{
+    fn2::ClearError();
+    if( !engine ) { fn2::SetError( "null engine" ); return 1; }
+    return static_cast<Engine*>( engine )->Wait( requestId, outputMinMax, latencyMs );
+}
+
+FASTNOISE_API int fn2ServeDrain( void* engine )
+{
+    fn2::ClearError();
+    if( !engine ) { fn2::SetError( "null engine" ); return 1; }
+    return static_cast<Engine*>( engine )->Drain();
+}
+
+FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* completed, int64_t* peakBytes )
+{
+    if( !engine ) return 1;
+    static_cast<Engine*>( engine )->Stats( batches, completed, peakBytes );
+    return 0;
+}
+
 }

@@ -1,0 +1,78 @@
+"""Python face of the serving engine (fn2Serve* in include/fn2_gpu.h):
+continuous batching of chunk requests against resident node trees."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import GenSpec, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, last_error
+
+__all__ = ["ServeEngine", "SERVE_PLAIN", "SERVE_BATCHED", "SERVE_GRAPH"]
+
+
+class ServeEngine:
+    def __init__(self, hbm_budget_bytes=16 << 30, max_batch_points=1 << 23, mode=SERVE_BATCHED):
+        self.lib = _lib.load()
+        self.h = self.lib.fn2ServeCreate(hbm_budget_bytes, max_batch_points, mode)
+        if not self.h:
+            raise RuntimeError("fn2ServeCreate failed: " + last_error())
+        self.mode = mode
+        self._outs = {}
+
+    def close(self):
+        if self.h:
+            self.lib.fn2ServeDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_tree(self, encoded):
+        if not isinstance(encoded, str):
+            encoded = encoded.encode()
+        t = self.lib.fn2ServeAddTree(self.h, encoded.encode())
+        if t < 0:
+            raise ValueError("fn2ServeAddTree failed: " + last_error())
+        return t
+
+    def submit(self, tree, spec, out=None):
+        """Queues one request; `out` (float32, spec's point count) receives the values."""
+        n = spec_points(spec)
+        if out is None:
+            out = np.empty(n, dtype=np.float32)
+        assert out.dtype == np.float32 and out.size >= n and out.flags.c_contiguous
+        rid = self.lib.fn2ServeSubmit(self.h, tree, ctypes.byref(spec), out.ctypes.data_as(_lib.c_float_p))
+        if rid < 0:
+            raise RuntimeError("fn2ServeSubmit failed: " + last_error())
+        self._outs[rid] = out
+        return rid
+
+    def wait(self, rid):
+        """(values, (min, max), latency in ms) of a submitted request."""
+        mm = np.zeros(2, dtype=np.float32)
+        lat = ctypes.c_double()
+        rc = self.lib.fn2ServeWait(self.h, rid, mm.ctypes.data_as(_lib.c_float_p), ctypes.byref(lat))
+        out = self._outs.pop(rid)
+        if rc:
+            raise RuntimeError("request %d failed: %s" % (rid, last_error()))
+        return out, (float(mm[0]), float(mm[1])), lat.value
+
+    def drain(self):
+        self.lib.fn2ServeDrain(self.h)
+
+    def stats(self):
+        b, c, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self.lib.fn2ServeStats(self.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(p))
+        return {"batches": b.value, "completed": c.value, "device_bytes": p.value}
+
+
+def spec_points(spec):
+    if spec.kind == _lib.SOURCE_TILEABLE:
+        return spec.count[0] * spec.count[1]
+    n = 1
+    for a in range(spec.dims):
+        n *= spec.count[a]
+    return n

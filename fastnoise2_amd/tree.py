@@ -311,3 +311,73 @@ def b64decode(text):
         n = 3 - quad.count("=")
         out += bytes([(t >> 16) & 255, (t >> 8) & 255, t & 255][:n])
     return bytes(out)
+
+
+def decode(encoded):
+    """Parses an encoded node tree into NodeSpecs (DeserialiseNodeData of the
+    reference, Metadata.cpp:311-353); shared nodes come back as the same
+    NodeSpec object. Raises ValueError on malformed input."""
+    data = b64decode(encoded)
+    pos = [0]
+    refs = []
+    pending_ends = [0]
+
+    def take(n):
+        if pending_ends[0] or pos[0] + n > len(data):
+            raise ValueError("truncated node stream")
+        v = data[pos[0]:pos[0] + n]
+        pos[0] += n
+        return v
+
+    def member():
+        if pending_ends[0]:
+            pending_ends[0] -= 1
+            return 4, 0
+        b = take(1)[0]
+        typ, idx = b & 7, b >> 3
+        if typ == 4:
+            pending_ends[0] = idx
+        return typ, idx
+
+    metas, _ = registry()
+
+    def node():
+        nid = take(1)[0]
+        if nid == 0xFF:
+            ref = struct.unpack("<H", take(2))[0]
+            if ref >= len(refs):
+                raise ValueError("bad node reference")
+            return refs[ref]
+        if nid >= len(metas):
+            raise ValueError("unknown node id %d" % nid)
+        spec = NodeSpec(metas[nid].name)
+        md = spec.meta
+        typ, idx = member()
+        while typ == 0:
+            bits = struct.unpack("<i", take(4))[0]
+            if idx < len(md.variables):
+                m = md.variables[idx]
+                spec.vars[idx] = struct.unpack("<f", struct.pack("<i", bits))[0] if m.type == 0 else bits
+            typ, idx = member()
+        if typ == 1:
+            for k in range(idx):
+                child = node()
+                if k < len(md.lookups):
+                    spec.lookups[k] = child
+            typ, idx = member()
+        while typ != 4:
+            if typ == 2:
+                child = node()
+                if idx < len(md.hybrids):
+                    spec.hybrids[idx] = child
+            elif typ == 3:
+                v = struct.unpack("<f", take(4))[0]
+                if idx < len(md.hybrids):
+                    spec.hybrids[idx] = v
+            else:
+                raise ValueError("bad member record")
+            typ, idx = member()
+        refs.append(spec)
+        return spec
+
+    return node()

@@ -1,0 +1,136 @@
+"""Serving engine on the GPU: every request's result is the same alone (one
+plain API call) and in the mix (continuously batched with requests of other
+trees, sizes, dimensions and seeds, submitted from several threads), bit for
+bit, in all three engine modes; batched results also match the oracle."""
+import threading
+
+import numpy as np
+import pytest
+
+import fastnoise2_amd as fn
+from fastnoise2_amd import workloads as W
+from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
+from fastnoise2_amd.tree import NodeSpec as S
+import oracle
+from conftest import assert_bitexact
+
+pytestmark = pytest.mark.gpu
+
+TREES = [
+    W.serving_tree(),
+    W.fbm_simplex_2d(),
+    W.warped_cellular_3d(),
+    S("FractalRidged", source=S("Perlin", feature_scale=40.0), octaves=4),
+    S("Terrace", source=S("CellularValue", feature_scale=20.0), step_count=6.0),
+]
+
+
+def make_requests(n, seed=0):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n):
+        t = int(rng.integers(len(TREES)))
+        kind = rng.choice(["grid3", "grid2", "tile", "grid4"], p=[0.45, 0.35, 0.1, 0.1])
+        s = int(rng.integers(-2**31, 2**31))
+        if kind == "grid3":
+            c = int(rng.choice([8, 16, 24, 32]))
+            o = [float(v) for v in rng.integers(-1000, 1000, 3) * c]
+            spec = fn.grid_spec(3, o, [c, c, c], seed=s)
+        elif kind == "grid2":
+            c = int(rng.choice([33, 64, 128, 200]))
+            o = [float(v) for v in rng.integers(-1000, 1000, 2) * c]
+            spec = fn.grid_spec(2, o, [c, c + 7], seed=s)
+        elif kind == "grid4":
+            spec = fn.grid_spec(4, [1.5, -3.0, 7.0, 2.5], [6, 5, 4, 3], seed=s)
+        else:
+            spec = fn.tileable_spec(int(rng.choice([32, 64])), 48, seed=s)
+        reqs.append((t, spec))
+    return reqs
+
+
+def alone(nodes, t, spec):
+    n = nodes[t]
+    if spec.kind == fn.SOURCE_TILEABLE:
+        return n.gen_tileable_2d(spec.count[0], spec.count[1], spec.step[0], spec.step[1], spec.seed)
+    d = spec.dims
+    gen = {2: n.gen_uniform_grid_2d, 3: n.gen_uniform_grid_3d, 4: n.gen_uniform_grid_4d}[d]
+    return gen(*list(spec.offset)[:d], *list(spec.count)[:d], *list(spec.step)[:d], seed=spec.seed)
+
+
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN])
+def test_alone_equals_in_mix(gpu, mode):
+    nodes = [fn.Node.from_encoded(t.encode()) for t in TREES]
+    reqs = make_requests(120, seed=mode)
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=mode)
+    tids = [eng.add_tree(t.encode()) for t in TREES]
+    ids = [None] * len(reqs)
+
+    def submitter(k):
+        for i in range(k, len(reqs), 4):
+            t, spec = reqs[i]
+            ids[i] = eng.submit(tids[t], spec)
+
+    threads = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    results = [eng.wait(i) for i in ids]
+    st = eng.stats()
+    assert st["completed"] == len(reqs)
+    if mode != SERVE_PLAIN:
+        assert st["batches"] < len(reqs)
+    for (t, spec), (out, mm, lat) in zip(reqs, results):
+        ref, ref_mm = alone(nodes, t, spec)
+        assert_bitexact(out, ref, "request alone vs in the mix")
+        assert mm == ref_mm
+        assert lat > 0
+    eng.close()
+
+
+def test_batched_matches_oracle(gpu):
+    reqs = make_requests(24, seed=11)
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=SERVE_BATCHED)
+    tids = [eng.add_tree(t.encode()) for t in TREES]
+    ids = [eng.submit(tids[t], spec) for t, spec in reqs]
+    trees = [oracle.Tree(t.encode()) for t in TREES]
+    for (t, spec), rid in zip(reqs, ids):
+        out, mm, _ = eng.wait(rid)
+        if spec.kind == fn.SOURCE_TILEABLE:
+            ref, ref_mm = trees[t].tileable(spec.count[0], spec.count[1], seed=spec.seed)
+        else:
+            d = spec.dims
+            ref, ref_mm = trees[t].grid(list(spec.offset)[:d], list(spec.count)[:d], list(spec.step)[:d], seed=spec.seed)
+        assert_bitexact(out, ref, "batched request vs oracle")
+        assert mm == ref_mm
+    eng.close()
+
+
+def test_graph_replays_stay_correct(gpu):
+    # the same batch shape replayed many times, with eager work in between
+    node = fn.Node.from_encoded(TREES[3].encode())
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 16, mode=SERVE_GRAPH)
+    tid = eng.add_tree(TREES[3].encode())
+    for rep in range(6):
+        spec = fn.grid_spec(3, [rep * 16.0, 0, 0], [16, 16, 16], seed=rep)
+        out, mm, _ = eng.wait(eng.submit(tid, spec))
+        ref, ref_mm = node.gen_uniform_grid_3d(rep * 16.0, 0, 0, 16, 16, 16, seed=rep)
+        assert_bitexact(out, ref, "graph replay %d" % rep)
+        assert mm == ref_mm
+    eng.close()
+
+
+def test_admission_rejects_oversized_and_bad_requests(gpu):
+    eng = ServeEngine(hbm_budget_bytes=1 << 28, max_batch_points=1 << 12, mode=SERVE_BATCHED)
+    tid = eng.add_tree(TREES[1].encode())
+    with pytest.raises(RuntimeError, match="capacity"):
+        eng.submit(tid, fn.grid_spec(2, [0, 0], [128, 128]))
+    with pytest.raises(RuntimeError, match="unknown tree"):
+        eng.submit(tid + 5, fn.grid_spec(2, [0, 0], [8, 8]))
+    with pytest.raises(ValueError):
+        eng.add_tree("not a tree")
+    out, mm, _ = eng.wait(eng.submit(tid, fn.grid_spec(2, [0, 0], [64, 64])))
+    assert np.isfinite(out).all()
+    eng.close()
+    with pytest.raises(RuntimeError, match="budget"):
+        ServeEngine(hbm_budget_bytes=1 << 20, max_batch_points=1 << 22, mode=SERVE_BATCHED)
