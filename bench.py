@@ -93,6 +93,39 @@ def cpu_baseline(wl, enc, target_s=12.0):
             "sample": "%s (%d points, %.1f s), CPU oracle 16-lane SIMD, %d threads" % (what, n, dt, threads)}
 
 
+def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
+    """Chunk latency and throughput of the serving engine on the serving tree:
+    the Poisson trace (open loop) for p50/p99 latency, and a saturating closed
+    loop (128 outstanding) of the same request mix for throughput, in the
+    plain (one launch + copy per chunk), batched and graph-replay modes."""
+    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
+    enc = W.serving_tree().encode()
+    specs, arrivals = W.serving_trace(trace_requests, rate)
+    pts = np.array([s.count[0] * s.count[1] * (s.count[2] if s.dims == 3 else 1) for s in specs], dtype=np.float64)
+    out = {"tree": enc, "trace": "Poisson %g req/s, %d requests, sizes 16^3/32^3/64^3/128^2/512^2" % (rate, trace_requests),
+           "budget_bytes": 16 << 30}
+    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH)):
+        eng = ServeEngine(16 << 30, 1 << 23, mode)
+        tid = eng.add_tree(enc)
+        eng.replay([tid] * 64, specs[:64], None, 16)   # compile + warm up
+        rec = {}
+        if name != "graph":
+            lat, wall = eng.replay([tid] * len(specs), specs, arrivals, 512)
+            rec.update({"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "mean_ms": round(float(lat.mean()), 3), "trace_wall_s": round(wall, 2)})
+        n = min(saturate_requests, len(specs))
+        lat, wall = eng.replay([tid] * n, specs[:n], None, 128)
+        rec.update({"saturated_Gpoints_s": round(float(pts[:n].sum()) / wall / 1e9, 3),
+                    "saturated_requests_s": round(n / wall, 1),
+                    "saturated_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                    "saturated_p99_ms": round(float(np.percentile(lat, 99)), 3),
+                    "batches": eng.stats()["batches"]})
+        eng.close()
+        out[name] = rec
+    out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +134,7 @@ def main():
     ap.add_argument("--workload", default="fbm_simplex_2d_grid_4096x4096")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a single MI355X; "
@@ -172,6 +206,8 @@ def main():
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(wl, enc, args.cpu_seconds)
         rec["gpu_vs_cpu"] = round(value / rec["cpu_baseline"]["value"], 1)
+    if not args.no_serving and args.workload == "fbm_simplex_2d_grid_4096x4096":
+        rec["serving"] = serving_bench(W)
     print(json.dumps(rec), flush=True)
 
 

@@ -106,6 +106,7 @@ GPU_EXT = {
     "fn2ServeWait": (i, [vp, ctypes.c_int64, c_float_p, c_double_p]),
     "fn2ServeDrain": (i, [vp]),
     "fn2ServeStats": (i, [vp, c_int64_p, c_int64_p, c_int64_p]),
+    "fn2ServeReplay": (i, [vp, i, c_int_p, ctypes.POINTER(GenSpec), c_double_p, i, c_double_p, c_double_p]),
 }
 
 _lib = None

@@ -98,6 +98,7 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArg
     float mn = INFINITY, mx = -INFINITY;
     const uint32_t total = (uint32_t)a.total;
     const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
+#pragma unroll 1
     for( int p = 0; p < PPT; p++ )
     {
         const uint32_t i = base + p * BLOCK;

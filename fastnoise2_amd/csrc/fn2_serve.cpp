@@ -580,4 +580,67 @@ FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* comple
     return 0;
 }
 
+FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,
+                                  int maxOutstanding, double* latencyMs, double* wallSec )
+{
+    fn2::ClearError();
+    if( !engine || count < 0 || !treeIds || !specs || !latencyMs || maxOutstanding < 1 )
+    {
+        fn2::SetError( "bad replay arguments" );
+        return 1;
+    }
+    Engine* e = static_cast<Engine*>( engine );
+    // output buffers: one per outstanding request, sized for the largest request
+    int64_t maxPoints = 0;
+    for( int i = 0; i < count; i++ )
+    {
+        fn2::GenArgs a{};
+        int64_t n = 0;
+        std::string err;
+        if( !fn2::FillSpecArgs( specs[i], a, n, err ) )
+        {
+            fn2::SetError( err );
+            return 1;
+        }
+        maxPoints = std::max( maxPoints, n );
+    }
+    std::vector<std::vector<float>> pool( (size_t)maxOutstanding, std::vector<float>( (size_t)std::max<int64_t>( maxPoints, 1 ) ) );
+    std::deque<std::pair<int64_t, int>> pending;   // (request id, pool slot), in submit order
+    std::vector<int> freeSlots;
+    for( int k = maxOutstanding - 1; k >= 0; k-- ) freeSlots.push_back( k );
+    std::vector<int> reqIndex;
+    auto t0 = Clock::now();
+    int status = 0;
+    int done = 0;
+    auto retireOne = [&]() {
+        auto p = pending.front();
+        pending.pop_front();
+        double lat = 0;
+        if( e->Wait( p.first, nullptr, &lat ) ) status = 1;
+        latencyMs[reqIndex[done]] = lat;
+        done++;
+        freeSlots.push_back( p.second );
+    };
+    for( int i = 0; i < count; i++ )
+    {
+        if( arrivalSec )
+        {
+            auto due = t0 + std::chrono::duration_cast<Clock::duration>( std::chrono::duration<double>( arrivalSec[i] ) );
+            // retire what has completed while waiting for the arrival time
+            while( !pending.empty() && Clock::now() < due && freeSlots.empty() ) retireOne();
+            std::this_thread::sleep_until( due );
+        }
+        while( freeSlots.empty() ) retireOne();
+        int slot = freeSlots.back();
+        freeSlots.pop_back();
+        int64_t id = e->Submit( treeIds[i], specs[i], pool[slot].data() );
+        if( id < 0 ) return 1;
+        pending.emplace_back( id, slot );
+        reqIndex.push_back( i );
+    }
+    while( !pending.empty() ) retireOne();
+    if( wallSec ) *wallSec = std::chrono::duration<double>( Clock::now() - t0 ).count();
+    return status;
+}
+
 }

@@ -63,6 +63,23 @@ class ServeEngine:
     def drain(self):
         self.lib.fn2ServeDrain(self.h)
 
+    def replay(self, tree_ids, specs, arrivals=None, max_outstanding=64):
+        """Native trace replay (fn2ServeReplay): returns (latencies in ms, wall seconds)."""
+        n = len(specs)
+        arr_specs = (GenSpec * n)(*specs)
+        tids = (ctypes.c_int * n)(*tree_ids)
+        lat = np.zeros(n, dtype=np.float64)
+        wall = ctypes.c_double()
+        arr = None
+        if arrivals is not None:
+            arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        rc = self.lib.fn2ServeReplay(self.h, n, tids, arr_specs,
+                                     arr.ctypes.data_as(_lib.c_double_p) if arr is not None else None,
+                                     max_outstanding, lat.ctypes.data_as(_lib.c_double_p), ctypes.byref(wall))
+        if rc:
+            raise RuntimeError("fn2ServeReplay failed: " + last_error())
+        return lat, wall.value
+
     def stats(self):
         b, c, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         self.lib.fn2ServeStats(self.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(p))

@@ -67,3 +67,26 @@ def splitmix64_positions(n, seed=7, lo=-1000.0, hi=1000.0):
             idx = k.astype(np.int64)
             out[idx % 3, idx // 3] = v
     return out[0], out[1], out[2]
+
+
+SERVING_SIZES = [("3d", 16), ("3d", 32), ("3d", 64), ("2d", 128), ("2d", 512)]
+
+
+def serving_trace(n=20000, rate=2000.0, seed=2024):
+    """Poisson request stream against the serving tree: mean `rate` requests/s;
+    each request has a uniform int32 seed, a size drawn uniformly from
+    16^3, 32^3, 64^3 (3D grids) and 128^2, 512^2 (2D grids), and an origin on a
+    chunk-aligned multiple in [-1e5, 1e5]. Returns (specs, arrival seconds)."""
+    from . import grid_spec
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    specs = []
+    for _ in range(n):
+        kind, c = SERVING_SIZES[int(rng.integers(len(SERVING_SIZES)))]
+        s = int(rng.integers(-2**31, 2**31))
+        d = 3 if kind == "3d" else 2
+        k = int(1e5 // c)
+        origin = [float(v * c) for v in rng.integers(-k, k + 1, d)]
+        specs.append(grid_spec(d, origin, [c] * d, seed=s))
+    return specs, arrivals

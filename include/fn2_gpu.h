@@ -100,8 +100,17 @@ FASTNOISE_API int fn2ServeWait( void* engine, int64_t requestId, float* outputMi
 /* Waits for everything submitted so far. */
 FASTNOISE_API int fn2ServeDrain( void* engine );
 
-/* Engine counters: batches launched, requests completed, peak device bytes. */
+/* Engine counters: batches launched, requests completed, device bytes held. */
 FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* completed, int64_t* peakBytes );
+
+/* Replays a request trace against the engine from the calling thread:
+ * request i (tree treeIds[i], spec specs[i]) is submitted at arrivalSec[i]
+ * seconds after the start (open loop; NULL = as fast as possible), with at
+ * most maxOutstanding requests in flight (closed-loop bound; outputs go to an
+ * internal buffer pool). latencyMs[i] receives request i's submit-to-done
+ * time, wallSec the time from the start to the last completion. */
+FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,
+                                  int maxOutstanding, double* latencyMs, double* wallSec );
 
 #ifdef __cplusplus
 }
