@@ -126,6 +126,39 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
     return out
 
 
+def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
+    """The mixed-tree serving workload: 8 seeded random trees (depth 3..12),
+    2D/3D/4D requests of 2^12..2^22 points, bursty on/off arrivals (10 ms at
+    10k req/s); per-tree tuned batched launches vs graph replay vs plain."""
+    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
+    encs = W.mixed_trees()
+    trees, specs, arrivals = W.mixed_trace(trace_requests)
+    pts = np.array([float(np.prod(list(s.count)[:s.dims])) for s in specs])
+    out = {"trees": len(encs), "trace": "on/off bursts 10 ms at 10k req/s, %d requests, 2^12..2^22 points, dims 2-4"
+           % trace_requests, "mean_points": round(float(pts.mean()))}
+    warm = [W.grid_spec_like(d) for d in (2, 3, 4)]
+    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH)):
+        eng = ServeEngine(16 << 30, 1 << 23, mode)
+        tids = [eng.add_tree(e) for e in encs]
+        wt = [t for t in tids for _ in warm]
+        eng.replay(wt, warm * len(tids), None, 8)   # compile + tune every tree and dimensionality
+        rec = {}
+        lat, wall = eng.replay([tids[t] for t in trees], specs, arrivals, 512)
+        rec.update({"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                    "trace_wall_s": round(wall, 2)})
+        n = min(saturate_requests, len(specs))
+        lat, wall = eng.replay([tids[t] for t in trees[:n]], specs[:n], None, 64)
+        rec.update({"saturated_Gpoints_s": round(float(pts[:n].sum()) / wall / 1e9, 3),
+                    "saturated_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                    "saturated_p99_ms": round(float(np.percentile(lat, 99)), 3)})
+        if mode != SERVE_PLAIN:
+            rec["tuned_points_per_lane"] = [[eng.tree_tuning(t, d) for d in (2, 3, 4)] for t in tids]
+        eng.close()
+        out[name] = rec
+    out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a single MI355X; "
@@ -145,6 +180,14 @@ def main():
     import fastnoise2_amd.workloads as W
     if fn.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src, dst = fn.DeviceArray(nbytes // 4), fn.DeviceArray(nbytes // 4)
+        for _ in range(3):
+            if fn.lib().fn2Memcpy(dst.ptr, src.ptr, nbytes, 2):
+                raise SystemExit("copy failed: " + fn.last_error())
+        print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+        return
     wls = workloads(W)
     wl = wls[args.workload]
     spec_tree = wl["tree"]()
@@ -203,11 +246,22 @@ def main():
         "config": {"workload": args.workload, "tree": enc, "points": n, "seed": wl["seed"]},
         "roofline": roofline,
     }
+    if wl["kind"] == "grid":
+        # the reference's own call shape: host output buffer, PCIe copy included
+        d = wl["dims"]
+        gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d}[d]
+        gen(*wl["offset"], *wl["count"], seed=wl["seed"])
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gen(*wl["offset"], *wl["count"], seed=wl["seed"])
+        rec["api_roundtrip_Gpoints_s"] = round(n * reps / (time.perf_counter() - t0) / 1e9, 3)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(wl, enc, args.cpu_seconds)
         rec["gpu_vs_cpu"] = round(value / rec["cpu_baseline"]["value"], 1)
     if not args.no_serving and args.workload == "fbm_simplex_2d_grid_4096x4096":
         rec["serving"] = serving_bench(W)
+        rec["serving_mixed"] = serving_mixed_bench(W)
     print(json.dumps(rec), flush=True)
 
 

@@ -98,6 +98,7 @@ GPU_EXT = {
     "fn2DeviceSynchronize": (i, []),
     "fn2DeviceCount": (i, []),
     "fn2GetLastError": (s, []),
+    "fn2SelfTestMath": (i, [i, vp, vp, i]),
     "fn2GetProgramInfo": (i, [vp, i, c_int_p, c_int_p, c_int_p, c_int_p]),
     "fn2ServeCreate": (vp, [ctypes.c_size_t, i, i]),
     "fn2ServeDestroy": (None, [vp]),
@@ -106,6 +107,7 @@ GPU_EXT = {
     "fn2ServeWait": (i, [vp, ctypes.c_int64, c_float_p, c_double_p]),
     "fn2ServeDrain": (i, [vp]),
     "fn2ServeStats": (i, [vp, c_int64_p, c_int64_p, c_int64_p]),
+    "fn2ServeTreeTuning": (i, [vp, i, i]),
     "fn2ServeReplay": (i, [vp, i, c_int_p, ctypes.POINTER(GenSpec), c_double_p, i, c_double_p, c_double_p]),
 }
 

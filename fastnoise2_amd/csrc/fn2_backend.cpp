@@ -29,6 +29,7 @@
 extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
+int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStream_t stream );
 }
 
 namespace fn2
@@ -55,6 +56,12 @@ namespace fn2
         std::map<std::tuple<int, int, uint32_t, uint32_t>, float*> gTileTables;
         constexpr size_t kMaxTileTables = 1024;
 
+        // Uploads of programs and tables go through one non-blocking stream:
+        // never through the legacy default stream, which would conflict with a
+        // serving engine capturing a graph on another thread.
+        std::mutex gUploadMu;
+        hipStream_t gUploadStream = nullptr;
+
         void DeviceInit()
         {
             int n = 0;
@@ -63,6 +70,13 @@ namespace fn2
             {
                 (void)hipGetLastError();
                 gInitError = "no HIP device available (the FastNoise2 MI355X backend runs on the GPU only)";
+                gInitStatus = 1;
+                return;
+            }
+            if( hipStreamCreateWithFlags( &gUploadStream, hipStreamNonBlocking ) != hipSuccess )
+            {
+                (void)hipGetLastError();
+                gInitError = "cannot create the upload stream";
                 gInitStatus = 1;
                 return;
             }
@@ -99,6 +113,14 @@ namespace fn2
         if( e == hipSuccess ) return true;
         SetError( std::string( what ) + ": " + hipGetErrorString( e ) );
         return false;
+    }
+
+    hipError_t UploadSync( void* dst, const void* src, size_t bytes )
+    {
+        std::lock_guard<std::mutex> lk( gUploadMu );
+        hipError_t e = hipMemcpyAsync( dst, src, bytes, hipMemcpyHostToDevice, gUploadStream );
+        if( e == hipSuccess ) e = hipStreamSynchronize( gUploadStream );
+        return e;
     }
 
     FastDiv MakeFastDiv( uint32_t d )
@@ -176,10 +198,10 @@ namespace fn2
             delete d;
             return nullptr;
         }
-        if( !p.ins.empty() && !HipOk( hipMemcpy( d->dIns, p.ins.data(), p.ins.size() * sizeof( Ins ), hipMemcpyHostToDevice ), "upload program" ) ) return nullptr;
+        if( !p.ins.empty() && !HipOk( UploadSync( d->dIns, p.ins.data(), p.ins.size() * sizeof( Ins ) ), "upload program" ) ) return nullptr;
         std::vector<float> padded( p.konst );
         padded.resize( p.konst.size() + 16, 0.0f );
-        if( !HipOk( hipMemcpy( d->dKonst, padded.data(), padded.size() * sizeof( float ), hipMemcpyHostToDevice ), "upload constants" ) ) return nullptr;
+        if( !HipOk( UploadSync( d->dKonst, padded.data(), padded.size() * sizeof( float ) ), "upload constants" ) ) return nullptr;
         bucket.push_back( d );
         gProgramCount++;
         return d;
@@ -320,7 +342,7 @@ namespace fn2
         }
         float* d = nullptr;
         hipError_t e = hipMalloc( (void**)&d, t.size() * sizeof( float ) );
-        if( e == hipSuccess ) e = hipMemcpy( d, t.data(), t.size() * sizeof( float ), hipMemcpyHostToDevice );
+        if( e == hipSuccess ) e = UploadSync( d, t.data(), t.size() * sizeof( float ) );
         if( e != hipSuccess )
         {
             err = std::string( "tileable table upload: " ) + hipGetErrorString( e );
@@ -761,6 +783,15 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
     if( msPerLaunch ) *msPerLaunch = ms / iters;
     if( wallMsPerLaunch ) *wallMsPerLaunch = std::chrono::duration<double, std::milli>( t1 - t0 ).count() / iters;
     return 0;
+}
+
+FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n )
+{
+    ClearError();
+    if( op < 0 || op > 3 || n < 0 ) { SetError( "bad self-test arguments" ); return 1; }
+    if( !EnsureDevice() ) return 1;
+    if( !HipOk( (hipError_t)fn2_launch_selftest_math( op, dIn, dOut, n, nullptr ), "self-test launch" ) ) return 1;
+    return HipOk( hipDeviceSynchronize(), "self-test" ) ? 0 : 1;
 }
 
 }

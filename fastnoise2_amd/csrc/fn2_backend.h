@@ -51,6 +51,8 @@ namespace fn2
     const char* LastError();
     bool EnsureDevice();
     bool HipOk( hipError_t e, const char* what );
+    // synchronous host-to-device copy on the library's non-blocking upload stream
+    hipError_t UploadSync( void* dst, const void* src, size_t bytes );
 
     FastDiv MakeFastDiv( uint32_t d );
     float DecodeKey( int32_t k );

@@ -102,7 +102,7 @@ namespace fn2d
                         int32_t h = cellHash<D>( cp.seed, hc );
                         float v[D];
                         cellVector<D>( h, v );
-                        float invMag = cp.jitter * invSqrt( cellMagSq<D>( v ) );
+                        float invMag = cp.jitter * invSqrtPos( cellMagSq<D>( v ) );
                         cell( h, v, invMag, fc );
                         if constexpr( D > 3 ) { fc[3] = fc[3] + 1.0f; hc[3] = iadd( hc[3], prime[3] ); }
                     }

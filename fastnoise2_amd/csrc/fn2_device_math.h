@@ -57,6 +57,41 @@ namespace fn2d
     // FS::Convert<int32_t> of a non-integer: round to nearest even
     FN2D int32_t roundToInt( float a ) { return (int32_t)__builtin_rintf( a ); }
     FN2D float invSqrt( float a ) { return 1.0f / __builtin_sqrtf( a ); }
+
+    // Correctly rounded sqrt and reciprocal for positive, normal, finite
+    // arguments with normal results, built from the hardware estimates
+    // (v_sqrt_f32 / v_rcp_f32, both within 1 ulp) plus fma corrections,
+    // without the zero / denormal / infinity scaling of the general IEEE
+    // routines. Used where the argument range is known: the cellular jitter
+    // vector's squared length, which lies in [0.5, 2.2e6] because every
+    // component is a half-integer. fn2SelfTestMath checks them against the
+    // IEEE results over whole binades.
+#if defined( __HIP_DEVICE_COMPILE__ )
+    FN2D float sqrtPosRN( float x )
+    {
+        const float s = __builtin_amdgcn_sqrtf( x );
+        const float down = ibits( fbits( s ) - 1 );
+        const float up = ibits( fbits( s ) + 1 );
+        // x - down * s <= 0: sqrt(x) lies below the midpoint of (down, s)
+        const float rDown = __builtin_fmaf( -down, s, x );
+        // x - up * s > 0: sqrt(x) lies above the midpoint of (s, up)
+        const float rUp = __builtin_fmaf( -up, s, x );
+        float r = rDown <= 0.0f ? down : s;
+        return rUp > 0.0f ? up : r;
+    }
+
+    FN2D float recipPosRN( float b )
+    {
+        const float y = __builtin_amdgcn_rcpf( b );
+        const float e = __builtin_fmaf( -b, y, 1.0f );   // exact residual
+        return __builtin_fmaf( e, y, y );
+    }
+#else
+    FN2D float sqrtPosRN( float x ) { return __builtin_sqrtf( x ); }
+    FN2D float recipPosRN( float b ) { return 1.0f / b; }
+#endif
+    // 1 / sqrt( x ) rounded twice, as FS::InvSqrt under strict semantics
+    FN2D float invSqrtPos( float x ) { return recipPosRN( sqrtPosRN( x ) ); }
     FN2D float recip( float a ) { return 1.0f / a; }
 
     // ---- hashing (Utils.inl:221-250)

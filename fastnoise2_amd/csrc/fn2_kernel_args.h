@@ -21,5 +21,7 @@ namespace fn2
         float offset[4];
         float step[4];
         const float* pos[4];     // position arrays (device); tileable: cos x, cos y, sin x, sin y tables
+        int32_t ppt;             // batched kernel: points per lane of each block of this job
+        int32_t reserved;
     };
 }

@@ -84,9 +84,10 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate( fn2::GenArgs a )
 // Batched form for the serving layer: every block looks up its request in a
 // per-block table, so requests with different trees, sizes and sources run in
 // one launch. `jobs` holds one GenArgs per request; block b covers points
-// [blockBase[b], blockBase[b] + BLOCK * PPT) of request blockJob[b] (clipped
-// to the request's size).
-template<int BLOCK, int PPT, int DM, int FT>
+// [blockBase[b], blockBase[b] + BLOCK * ppt) of request blockJob[b] (clipped
+// to the request's size), where ppt is the job's points per lane (chosen per
+// tree by the serving engine's tuner).
+template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ blockJob, const uint32_t* __restrict__ blockBase )
 {
     // read the job through a pointer (uniform address -> scalar loads) rather
@@ -98,8 +99,9 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArg
     float mn = INFINITY, mx = -INFINITY;
     const uint32_t total = (uint32_t)a.total;
     const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
+    const int ppt = a.ppt;
 #pragma unroll 1
-    for( int p = 0; p < PPT; p++ )
+    for( int p = 0; p < ppt; p++ )
     {
         const uint32_t i = base + p * BLOCK;
         if( i < total )
@@ -117,6 +119,18 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArg
     if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
 }
 
+// Applies one of the device math routines to an array (op 0: sqrtPosRN,
+// 1: recipPosRN, 2: invSqrtPos, 3: the IEEE 1 / sqrt).
+__global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if( i < n )
+    {
+        float x = in[i];
+        out[i] = op == 0 ? sqrtPosRN( x ) : op == 1 ? recipPosRN( x ) : op == 2 ? invSqrtPos( x ) : invSqrt( x );
+    }
+}
+
 __global__ void fn2_minmax_init( int32_t* keys, int n )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -130,7 +144,6 @@ __global__ void fn2_minmax_init( int32_t* keys, int n )
 namespace
 {
     constexpr int kBlock = 256;
-    constexpr int kBatchPPT = 4;
 
     // Blocks of the single-call kernel: enough to fill every CU several times
     // over, few enough that the per-block range atomics stay negligible.
@@ -163,11 +176,11 @@ extern "C" {
     X( 4, 7 ) X( 8, 7 ) X( 16, 7 ) \
     X( 28, 7 )
 
-// points per block of the batched kernel (the serving layer places requests
-// on multiples of this)
-int fn2_batch_tile_points( void )
+// threads per block of the batched kernel: a job with ppt points per lane
+// covers kBlock * ppt points per block
+int fn2_batch_block_threads( void )
 {
-    return kBlock * kBatchPPT;
+    return kBlock;
 }
 
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream )
@@ -191,12 +204,19 @@ int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJ
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        hipLaunchKernelGGL( ( fn2_generate_batch<kBlock, kBatchPPT, DMV, FTV> ), dim3( nBlocks ), dim3( kBlock ), 0, stream, dJobs, dBlockJob, dBlockBase ); \
+        hipLaunchKernelGGL( ( fn2_generate_batch<kBlock, DMV, FTV> ), dim3( nBlocks ), dim3( kBlock ), 0, stream, dJobs, dBlockJob, dBlockBase ); \
         return (int)hipGetLastError(); \
     }
     FN2_VARIANTS( FN2_TRY )
 #undef FN2_TRY
     return (int)hipErrorInvalidValue;
+}
+
+int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStream_t stream )
+{
+    if( n <= 0 ) return 0;
+    hipLaunchKernelGGL( fn2_selftest_math, dim3( ( n + 255 ) / 256 ), dim3( 256 ), 0, stream, op, in, out, n );
+    return (int)hipGetLastError();
 }
 
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream )

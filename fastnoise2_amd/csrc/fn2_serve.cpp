@@ -45,7 +45,7 @@ extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
 int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask, hipStream_t stream );
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
-int fn2_batch_tile_points( void );
+int fn2_batch_block_threads( void );
 }
 
 namespace
@@ -53,8 +53,11 @@ namespace
     using Clock = std::chrono::steady_clock;
     using fn2::GenArgs;
 
-    // points per block of the batched kernel; requests start on multiples of it
-    const int kTile = fn2_batch_tile_points();
+    // threads per block of the batched kernel; requests start on multiples of it
+    const int kThreads = fn2_batch_block_threads();
+    // candidate points per lane of the tuner, and its calibration size
+    constexpr int kPptCandidates[] = { 1, 2, 4, 8 };
+    constexpr int kTunePoints = 1 << 20;
     constexpr int kSlots = 3;
     constexpr int kMaxJobs = 4096;
 
@@ -63,6 +66,7 @@ namespace
         FastNoise::SmartNode<> node;
         const fn2::DeviceProgram* prog[5] = {};
         std::string compileError[5];
+        int ppt[5] = {};   // tuned points per lane of the batched kernel, per dimensionality
     };
 
     struct Request
@@ -78,6 +82,7 @@ namespace
         // placement in its batch slot
         size_t outOffset = 0;
         int job = 0;
+        int ppt = 1;
         // results
         bool done = false;
         int status = 0;
@@ -123,9 +128,9 @@ namespace
 
         bool Init()
         {
-            if( capacity < kTile ) capacity = kTile;
-            capacity = (int64_t)AlignUp( (size_t)capacity, kTile );
-            maxBlocks = (int)( capacity / kTile ) + kMaxJobs;
+            if( capacity < kThreads ) capacity = kThreads;
+            capacity = (int64_t)AlignUp( (size_t)capacity, kThreads );
+            maxBlocks = (int)( capacity / kThreads ) + kMaxJobs;
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
@@ -156,6 +161,21 @@ namespace
                 s.dBlockBase = (uint32_t*)( (uint8_t*)s.dJobs + jobsBytes + bjBytes );
             }
             deviceBytes = (int64_t)( perSlot * nSlots );
+            if( mode != FN2_SERVE_PLAIN )
+            {
+                size_t tb = (size_t)kTunePoints / kThreads + 1;
+                if( !fn2::HipOk( hipStreamCreateWithFlags( &tuneStream, hipStreamNonBlocking ), "tune stream" ) ||
+                    !fn2::HipOk( hipEventCreate( &tuneEv0 ), "tune event" ) || !fn2::HipOk( hipEventCreate( &tuneEv1 ), "tune event" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&tuneOut, (size_t)kTunePoints * 4 ), "tune arena" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&tuneJob, sizeof( GenArgs ) ), "tune job" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&tuneKeys, 2 * sizeof( int32_t ) ), "tune keys" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&tuneBJ, tb * sizeof( int32_t ) * 2 ), "tune tables" ) )
+                {
+                    initError = fn2::LastError();
+                    return false;
+                }
+                deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + 8 + tb * 8 );
+            }
             dispatcher = std::thread( [this] { DispatchLoop(); } );
             completer = std::thread( [this] { CompleteLoop(); } );
             return true;
@@ -187,6 +207,21 @@ namespace
                 if( s.stream ) (void)hipStreamDestroy( s.stream );
             }
             for( auto& kv : requests ) delete kv.second;
+            if( tuneStream ) (void)hipStreamSynchronize( tuneStream );
+            if( tuneOut ) (void)hipFree( tuneOut );
+            if( tuneJob ) (void)hipFree( tuneJob );
+            if( tuneKeys ) (void)hipFree( tuneKeys );
+            if( tuneBJ ) (void)hipFree( tuneBJ );
+            if( tuneEv0 ) (void)hipEventDestroy( tuneEv0 );
+            if( tuneEv1 ) (void)hipEventDestroy( tuneEv1 );
+            if( tuneStream ) (void)hipStreamDestroy( tuneStream );
+        }
+
+        int TreePpt( int treeId, int dims )
+        {
+            std::lock_guard<std::mutex> lk( mu );
+            if( treeId < 0 || treeId >= (int)trees.size() || dims < 2 || dims > 4 ) return -1;
+            return trees[treeId]->ppt[dims];
         }
 
         int AddTree( const char* encoded )
@@ -250,6 +285,15 @@ namespace
                     return -1;
                 }
                 prog = t.prog[d];
+                r->ppt = t.ppt[d];
+            }
+            if( r->ppt == 0 )
+            {
+                int best = mode == FN2_SERVE_PLAIN ? 1 : Tune( prog, r->args.dims );
+                if( best <= 0 ) return -1;
+                std::lock_guard<std::mutex> lk( mu );
+                trees[treeId]->ppt[r->args.dims] = best;
+                r->ppt = best;
             }
             r->prog = prog;
             r->args.ins = prog->dIns;
@@ -309,6 +353,86 @@ namespace
         }
 
     private:
+        // Per-tree tuning of the batched kernel: times a 2^20-point request of
+        // the program at each candidate points-per-lane on the tuning stream
+        // and keeps the fastest. Runs once per tree and dimensionality, in the
+        // thread that submits the tree's first request of that dimensionality.
+        int Tune( const fn2::DeviceProgram* prog, int dims )
+        {
+            std::lock_guard<std::mutex> lk( tuneMu );
+            fn2GenSpec spec{};
+            spec.kind = FN2_SOURCE_GRID;
+            spec.dims = dims;
+            int side = dims == 2 ? 1024 : dims == 3 ? 128 : 32;
+            for( int a = 0; a < dims; a++ )
+            {
+                spec.count[a] = side;
+                spec.step[a] = 1.0f;
+            }
+            if( dims == 3 ) spec.count[2] = kTunePoints / ( side * side );
+            GenArgs a{};
+            int64_t n = 0;
+            std::string err;
+            if( !fn2::FillSpecArgs( spec, a, n, err ) )
+            {
+                fn2::SetError( err );
+                return -1;
+            }
+            a.ins = prog->dIns;
+            a.konst = prog->dKonst;
+            a.nIns = (int32_t)prog->ins.size();
+            a.resultReg = prog->resultReg;
+            a.out = tuneOut;
+            a.minmax = tuneKeys;
+            int best = 1;
+            float bestMs = 1e30f;
+            std::vector<int32_t> bj;
+            std::vector<uint32_t> bb;
+            size_t tb = (size_t)kTunePoints / kThreads + 1;
+            for( int ppt : kPptCandidates )
+            {
+                a.ppt = ppt;
+                const int64_t tile = (int64_t)kThreads * ppt;
+                int nb = (int)( ( n + tile - 1 ) / tile );
+                bj.assign( nb, 0 );
+                bb.resize( nb );
+                for( int b = 0; b < nb; b++ ) bb[b] = (uint32_t)( b * tile );
+                int32_t* dBJ = tuneBJ;
+                uint32_t* dBB = (uint32_t*)( tuneBJ + tb );
+                if( !fn2::HipOk( fn2::UploadSync( tuneJob, &a, sizeof( a ) ), "tune upload" ) ||
+                    !fn2::HipOk( fn2::UploadSync( dBJ, bj.data(), nb * sizeof( int32_t ) ), "tune upload" ) ||
+                    !fn2::HipOk( fn2::UploadSync( dBB, bb.data(), nb * sizeof( uint32_t ) ), "tune upload" ) )
+                    return -1;
+                float ms = 1e30f;
+                for( int rep = 0; rep < 3; rep++ )
+                {
+                    if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, tuneStream ), "tune launch" ) ||
+                        !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
+                        !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, tuneStream ), "tune launch" ) ||
+                        !fn2::HipOk( hipEventRecord( tuneEv1, tuneStream ), "tune event" ) ||
+                        !fn2::HipOk( hipEventSynchronize( tuneEv1 ), "tune run" ) )
+                        return -1;
+                    float t = 0;
+                    if( !fn2::HipOk( hipEventElapsedTime( &t, tuneEv0, tuneEv1 ), "tune timing" ) ) return -1;
+                    if( rep > 0 ) ms = std::min( ms, t );   // first run warms caches
+                }
+                if( ms < bestMs * 0.97f )
+                {
+                    bestMs = ms;
+                    best = ppt;
+                }
+            }
+            return best;
+        }
+
+        std::mutex tuneMu;
+        hipStream_t tuneStream = nullptr;
+        hipEvent_t tuneEv0 = nullptr, tuneEv1 = nullptr;
+        float* tuneOut = nullptr;
+        GenArgs* tuneJob = nullptr;
+        int32_t* tuneKeys = nullptr;
+        int32_t* tuneBJ = nullptr;
+
         std::mutex mu;
         std::condition_variable cvWork, cvDone, cvSlot;
         bool stop = false;
@@ -336,8 +460,8 @@ namespace
             {
                 if( mode == FN2_SERVE_PLAIN && !s.reqs.empty() ) break;
                 Request* r = queue.front();
-                // block-aligned placement keeps every block inside one request
-                int64_t span = ( r->points + kTile - 1 ) / kTile * kTile;
+                // requests start on whole blocks of threads (1 KiB) in the arena
+                int64_t span = ( r->points + kThreads - 1 ) / kThreads * kThreads;
                 if( s.points + span > capacity ) break;
                 queue.pop_front();
                 r->outOffset = (size_t)s.points;
@@ -366,13 +490,15 @@ namespace
                 r->job = (int)j;
                 GenArgs a = r->args;
                 a.out = s.dOut + r->outOffset;
+                a.ppt = r->ppt;
                 a.minmax = s.dKeys + 2 * j;
                 hJobs[j] = a;
-                int rb = (int)( ( r->points + kTile - 1 ) / kTile );
+                const int64_t tile = (int64_t)kThreads * r->ppt;
+                int rb = (int)( ( r->points + tile - 1 ) / tile );
                 for( int b = 0; b < rb; b++ )
                 {
                     hBJ[nb + b] = (int32_t)j;
-                    hBB[nb + b] = (uint32_t)b * kTile;
+                    hBB[nb + b] = (uint32_t)( b * tile );
                 }
                 int key = VariantKey( r->prog );
                 if( shape.empty() || shape.back().first != key ) shape.emplace_back( key, rb );
@@ -428,8 +554,8 @@ namespace
                 if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, s.dBlockJob + first, s.dBlockBase + first, g.second, dm, fm, s.stream ), "batch launch" ) ) return false;
                 first += g.second;
             }
-            // every request occupies whole blocks from offset 0: nb blocks cover them all
-            size_t outBytes = (size_t)nb * kTile * 4;
+            // the requests occupy [0, points) of the arena
+            size_t outBytes = (size_t)s.points * 4;
             if( !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, outBytes, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
             if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, 2 * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
             return true;
@@ -578,6 +704,12 @@ FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* comple
     if( !engine ) return 1;
     static_cast<Engine*>( engine )->Stats( batches, completed, peakBytes );
     return 0;
+}
+
+FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims )
+{
+    if( !engine ) return -1;
+    return static_cast<Engine*>( engine )->TreePpt( treeId, dims );
 }
 
 FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,

@@ -80,6 +80,10 @@ class ServeEngine:
             raise RuntimeError("fn2ServeReplay failed: " + last_error())
         return lat, wall.value
 
+    def tree_tuning(self, tree, dims):
+        """Points per lane chosen for the tree's batched launches (0 = not tuned yet)."""
+        return self.lib.fn2ServeTreeTuning(self.h, tree, dims)
+
     def stats(self):
         b, c, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         self.lib.fn2ServeStats(self.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(p))

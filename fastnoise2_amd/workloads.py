@@ -90,3 +90,89 @@ def serving_trace(n=20000, rate=2000.0, seed=2024):
         origin = [float(v * c) for v in rng.integers(-k, k + 1, d)]
         specs.append(grid_spec(d, origin, [c] * d, seed=s))
     return specs, arrivals
+
+
+# ---------------------------------------------------------------- mixed trees
+_LEAVES = ["Perlin", "Simplex", "SuperSimplex", "Value", "CellularValue", "CellularDistance"]
+
+
+def random_tree(rng, depth):
+    """A seeded random node tree of about `depth` levels built from lattice and
+    cellular leaves, FBm/Ridged, Add/Multiply/MinSmooth/Fade, Remap/Terrace,
+    DomainScale/DomainRotate and domain warps."""
+    def leaf():
+        name = _LEAVES[int(rng.integers(len(_LEAVES)))]
+        kw = {"feature_scale": float(rng.choice([20.0, 40.0, 80.0, 150.0])), "seed_offset": int(rng.integers(0, 5))}
+        if name.startswith("Cellular"):
+            kw["distance_function"] = str(rng.choice(["Euclidean", "Euclidean Squared", "Manhattan", "Hybrid"]))
+        return S(name, **kw)
+
+    def build(d):
+        if d <= 1:
+            return leaf()
+        kind = int(rng.integers(8))
+        if kind == 0:
+            return S(str(rng.choice(["FractalFBm", "FractalRidged"])), source=leaf(), octaves=int(rng.integers(2, 6)))
+        if kind == 1:
+            return S(str(rng.choice(["Add", "Multiply"])), lhs=build(d - 1), rhs=build(d - 2))
+        if kind == 2:
+            return S("MinSmooth", lhs=build(d - 1), rhs=build(d - 2), smoothness=float(rng.uniform(0.05, 0.5)))
+        if kind == 3:
+            return S("Fade", a=build(d - 1), b=build(d - 2), fade=build(d - 2))
+        if kind == 4:
+            return S("Remap", source=build(d - 1), from_min=-1.0, from_max=1.0, to_min=0.0, to_max=1.0)
+        if kind == 5:
+            return S("Terrace", source=build(d - 1), step_count=float(rng.integers(2, 9)), smoothness=float(rng.uniform(0, 1)))
+        if kind == 6:
+            if rng.integers(2):
+                return S("DomainScale", source=build(d - 1), scaling=float(rng.uniform(0.5, 2.0)))
+            return S("DomainRotate", source=build(d - 1), yaw=float(rng.uniform(0, 3)), pitch=float(rng.uniform(0, 1)))
+        warp = str(rng.choice(["DomainWarpGradient", "DomainWarpSimplex", "DomainWarpSuperSimplex"]))
+        w = S(warp, source=build(d - 1), warp_amplitude=float(rng.uniform(5, 40)), feature_scale=float(rng.choice([50.0, 100.0])))
+        if rng.integers(2):
+            return S(str(rng.choice(["DomainWarpFractalProgressive", "DomainWarpFractalIndependent"])),
+                     domain_warp_source=w, octaves=int(rng.integers(2, 4)))
+        return w
+
+    return build(depth)
+
+
+def mixed_trees(seed=5, count=8):
+    """The 8 mixed-workload trees (depths 3..12), as encoded strings."""
+    rng = np.random.default_rng(seed)
+    depths = np.linspace(3, 12, count).round().astype(int)
+    return [random_tree(rng, int(d)).encode() for d in depths]
+
+
+def mixed_trace(n=4000, seed=77, burst_ms=10.0, burst_rate=10000.0, off_mean_ms=40.0, ntrees=8):
+    """Bursty requests over the mixed trees: on/off arrivals (bursts of
+    `burst_ms` at `burst_rate` requests/s, exponential off periods), uniform
+    tree, dimension 2/3/4, size log-uniform in [2^12, 2^22] points (a grid of
+    side round(N^(1/d))), uniform seed and origin. Returns (tree indices,
+    specs, arrival seconds)."""
+    from . import grid_spec
+    rng = np.random.default_rng(seed)
+    arrivals = []
+    t = 0.0
+    while len(arrivals) < n:
+        end = t + burst_ms / 1000.0
+        while t < end and len(arrivals) < n:
+            arrivals.append(t)
+            t += rng.exponential(1.0 / burst_rate)
+        t = end + rng.exponential(off_mean_ms / 1000.0)
+    trees, specs = [], []
+    for _ in range(n):
+        d = int(rng.integers(2, 5))
+        pts = 2.0 ** rng.uniform(12, 22)
+        side = max(2, int(round(pts ** (1.0 / d))))
+        origin = [float(v) for v in rng.integers(-100000, 100000, d)]
+        trees.append(int(rng.integers(ntrees)))
+        specs.append(grid_spec(d, origin, [side] * d, seed=int(rng.integers(-2**31, 2**31))))
+    return trees, specs, np.array(arrivals)
+
+
+def grid_spec_like(dims, points=1 << 14, seed=3):
+    """A small grid request of `dims` dimensions (warm-up of compile and tuning)."""
+    from . import grid_spec
+    side = max(2, int(round(points ** (1.0 / dims))))
+    return grid_spec(dims, [0.0] * dims, [side] * dims, seed=seed)

@@ -62,6 +62,12 @@ FASTNOISE_API int fn2DeviceCount( void );
 /* Last error of this thread ("" when none). */
 FASTNOISE_API const char* fn2GetLastError( void );
 
+/* Runs one of the library's device math routines over dIn[0..n) into dOut
+ * (device arrays): 0 sqrt, 1 reciprocal, 2 1/sqrt (the fast correctly rounded
+ * forms the cellular nodes use for positive normal arguments), 3 the IEEE
+ * 1/sqrt. Lets a test compare them with IEEE results on the installed GPU. */
+FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n );
+
 /* Compiled-program facts for a tree evaluated in `dims` dimensions. */
 FASTNOISE_API int fn2GetProgramInfo( const void* node, int dims, int* instructions, int* registers, int* dimMask, int* featureMask );
 
@@ -102,6 +108,12 @@ FASTNOISE_API int fn2ServeDrain( void* engine );
 
 /* Engine counters: batches launched, requests completed, device bytes held. */
 FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* completed, int64_t* peakBytes );
+
+/* Points per lane the engine's tuner chose for a tree's batched launches in
+ * `dims` dimensions (1, 2, 4 or 8; 0 = not tuned yet, -1 = bad arguments). The
+ * tuner times a 2^20-point request of the tree at each candidate on the
+ * first request of that tree and dimensionality. */
+FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims );
 
 /* Replays a request trace against the engine from the calling thread:
  * request i (tree treeIds[i], spec specs[i]) is submitted at arrivalSec[i]

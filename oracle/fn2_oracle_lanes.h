@@ -23,6 +23,9 @@
 #include <cstdint>
 #include <cstring>
 #include <cmath>
+#if defined( __AVX2__ ) || defined( __AVX512F__ )
+#include <immintrin.h>
+#endif
 
 #ifdef FN2O_COUNTING
 // Op-counting build (one lane): every vector operation the reference's SIMD
@@ -38,115 +41,205 @@ namespace fn2o_count
 #define FN2O_CV() ( (void)0 )
 #endif
 
+#ifndef FN2O_INLINE
+#define FN2O_INLINE inline __attribute__( ( always_inline ) )
+#endif
+
 namespace FN2O_NS
 {
 #define FN2O_LOOP for( int k = 0; k < N; k++ )
 
-    template<int N> struct FV { float v[N]; };
-    template<int N> struct IV { int32_t v[N]; };
+    // Lanes are GCC vector types, so a 16-lane value is one AVX-512 register
+    // (two AVX2 registers) and lane-wise operators compile to vector
+    // instructions; element access is still .v[k].
+    template<int N> struct FV
+    {
+        typedef float T __attribute__( ( vector_size( N * 4 ) ) );
+        T v;
+    };
+    template<int N> struct IV
+    {
+        typedef int32_t T __attribute__( ( vector_size( N * 4 ) ) );
+        typedef uint32_t U __attribute__( ( vector_size( N * 4 ) ) );
+        T v;
+    };
+    template<int N> FN2O_INLINE FV<N> mkf( typename FV<N>::T v ) { FV<N> r; r.v = v; return r; }
+    template<int N> FN2O_INLINE IV<N> mki( typename IV<N>::T v ) { IV<N> r; r.v = v; return r; }
+    template<int N> FN2O_INLINE typename IV<N>::U uv( IV<N> a ) { return (typename IV<N>::U)a.v; }
 
     static inline uint32_t bits_of( float f ) { uint32_t u; std::memcpy( &u, &f, 4 ); return u; }
     static inline float float_of( uint32_t u ) { float f; std::memcpy( &f, &u, 4 ); return f; }
 
-    template<int N> inline FV<N> fsplat( float a ) { FV<N> r; FN2O_LOOP r.v[k] = a; return r; }
-    template<int N> inline IV<N> isplat( int32_t a ) { IV<N> r; FN2O_LOOP r.v[k] = a; return r; }
+    template<int N> FN2O_INLINE FV<N> fsplat( float a ) { FV<N> r; r.v = a - (typename FV<N>::T){}; return r; }
+    template<int N> FN2O_INLINE IV<N> isplat( int32_t a ) { IV<N> r; r.v = a - (typename IV<N>::T){}; return r; }
 
     // ---- float arithmetic
-    template<int N> inline FV<N> operator+( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); FV<N> r; FN2O_LOOP r.v[k] = a.v[k] + b.v[k]; return r; }
-    template<int N> inline FV<N> operator-( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); FV<N> r; FN2O_LOOP r.v[k] = a.v[k] - b.v[k]; return r; }
-    template<int N> inline FV<N> operator*( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); FV<N> r; FN2O_LOOP r.v[k] = a.v[k] * b.v[k]; return r; }
-    template<int N> inline FV<N> operator/( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); FV<N> r; FN2O_LOOP r.v[k] = a.v[k] / b.v[k]; return r; }
-    template<int N> inline FV<N> operator-( FV<N> a ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = -a.v[k]; return r; }
-    template<int N> inline FV<N> operator+( FV<N> a, float b ) { return a + fsplat<N>( b ); }
-    template<int N> inline FV<N> operator-( FV<N> a, float b ) { return a - fsplat<N>( b ); }
-    template<int N> inline FV<N> operator*( FV<N> a, float b ) { return a * fsplat<N>( b ); }
-    template<int N> inline FV<N> operator-( float a, FV<N> b ) { return fsplat<N>( a ) - b; }
-    template<int N> inline FV<N> operator+( float a, FV<N> b ) { return fsplat<N>( a ) + b; }
-    template<int N> inline FV<N> operator*( float a, FV<N> b ) { return fsplat<N>( a ) * b; }
-    template<int N> inline FV<N>& operator+=( FV<N>& a, FV<N> b ) { a = a + b; return a; }
-    template<int N> inline FV<N>& operator-=( FV<N>& a, FV<N> b ) { a = a - b; return a; }
-    template<int N> inline FV<N>& operator*=( FV<N>& a, FV<N> b ) { a = a * b; return a; }
+    template<int N> FN2O_INLINE FV<N> operator+( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); return mkf<N>( a.v + b.v ); }
+    template<int N> FN2O_INLINE FV<N> operator-( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); return mkf<N>( a.v - b.v ); }
+    template<int N> FN2O_INLINE FV<N> operator*( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); return mkf<N>( a.v * b.v ); }
+    template<int N> FN2O_INLINE FV<N> operator/( FV<N> a, FV<N> b ) { FN2O_CF( 1 ); return mkf<N>( a.v / b.v ); }
+    template<int N> FN2O_INLINE FV<N> operator-( FV<N> a ) { FN2O_CV(); return mkf<N>( -a.v ); }
+    template<int N> FN2O_INLINE FV<N> operator+( FV<N> a, float b ) { return a + fsplat<N>( b ); }
+    template<int N> FN2O_INLINE FV<N> operator-( FV<N> a, float b ) { return a - fsplat<N>( b ); }
+    template<int N> FN2O_INLINE FV<N> operator*( FV<N> a, float b ) { return a * fsplat<N>( b ); }
+    template<int N> FN2O_INLINE FV<N> operator-( float a, FV<N> b ) { return fsplat<N>( a ) - b; }
+    template<int N> FN2O_INLINE FV<N> operator+( float a, FV<N> b ) { return fsplat<N>( a ) + b; }
+    template<int N> FN2O_INLINE FV<N> operator*( float a, FV<N> b ) { return fsplat<N>( a ) * b; }
+    template<int N> FN2O_INLINE FV<N>& operator+=( FV<N>& a, FV<N> b ) { a = a + b; return a; }
+    template<int N> FN2O_INLINE FV<N>& operator-=( FV<N>& a, FV<N> b ) { a = a - b; return a; }
+    template<int N> FN2O_INLINE FV<N>& operator*=( FV<N>& a, FV<N> b ) { a = a * b; return a; }
 
-    template<int N> inline FV<N> vfma( FV<N> a, FV<N> b, FV<N> c ) { FN2O_CF( 2 ); FV<N> r; FN2O_LOOP r.v[k] = fmaf( a.v[k], b.v[k], c.v[k] ); return r; }
-    template<int N> inline FV<N> vfloor( FV<N> a ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = floorf( a.v[k] ); return r; }
-    template<int N> inline FV<N> vround( FV<N> a ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = rintf( a.v[k] ); return r; }
-    template<int N> inline FV<N> vabs( FV<N> a ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = fabsf( a.v[k] ); return r; }
-    template<int N> inline FV<N> vsqrt( FV<N> a ) { FN2O_CF( 1 ); FV<N> r; FN2O_LOOP r.v[k] = sqrtf( a.v[k] ); return r; }
-    template<int N> inline FV<N> vmin( FV<N> a, FV<N> b ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = a.v[k] < b.v[k] ? a.v[k] : b.v[k]; return r; }
-    template<int N> inline FV<N> vmax( FV<N> a, FV<N> b ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = a.v[k] > b.v[k] ? a.v[k] : b.v[k]; return r; }
+    // fused multiply-add, floor, round-to-nearest-even, sqrt: x86 vector
+    // instructions in the 16-lane builds, libm per lane otherwise
+    template<int N> FN2O_INLINE FV<N> vfma( FV<N> a, FV<N> b, FV<N> c )
+    {
+        FN2O_CF( 2 );
+#if defined( __AVX512F__ )
+        if constexpr( N == 16 ) return mkf<N>( (typename FV<N>::T)_mm512_fmadd_ps( (__m512)a.v, (__m512)b.v, (__m512)c.v ) );
+#endif
+#if defined( __FMA__ ) && defined( __AVX2__ )
+        if constexpr( N == 16 )
+        {
+            FV<N> r;
+            __m256 lo = _mm256_fmadd_ps( _mm256_loadu_ps( (const float*)&a.v ), _mm256_loadu_ps( (const float*)&b.v ), _mm256_loadu_ps( (const float*)&c.v ) );
+            __m256 hi = _mm256_fmadd_ps( _mm256_loadu_ps( (const float*)&a.v + 8 ), _mm256_loadu_ps( (const float*)&b.v + 8 ), _mm256_loadu_ps( (const float*)&c.v + 8 ) );
+            _mm256_storeu_ps( (float*)&r.v, lo );
+            _mm256_storeu_ps( (float*)&r.v + 8, hi );
+            return r;
+        }
+#endif
+        FV<N> r;
+        FN2O_LOOP r.v[k] = fmaf( a.v[k], b.v[k], c.v[k] );
+        return r;
+    }
+    template<int N> FN2O_INLINE FV<N> vfloor( FV<N> a )
+    {
+        FN2O_CV();
+#if defined( __AVX512F__ )
+        if constexpr( N == 16 ) return mkf<N>( (typename FV<N>::T)_mm512_roundscale_ps( (__m512)a.v, _MM_FROUND_TO_NEG_INF | _MM_FROUND_NO_EXC ) );
+#endif
+        FV<N> r;
+        FN2O_LOOP r.v[k] = floorf( a.v[k] );
+        return r;
+    }
+    template<int N> FN2O_INLINE FV<N> vround( FV<N> a )
+    {
+        FN2O_CV();
+#if defined( __AVX512F__ )
+        if constexpr( N == 16 ) return mkf<N>( (typename FV<N>::T)_mm512_roundscale_ps( (__m512)a.v, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC ) );
+#endif
+        FV<N> r;
+        FN2O_LOOP r.v[k] = rintf( a.v[k] );
+        return r;
+    }
+    template<int N> FN2O_INLINE FV<N> vabs( FV<N> a ) { FN2O_CV(); return mkf<N>( (typename FV<N>::T)( (typename IV<N>::T)a.v & 0x7fffffff ) ); }
+    template<int N> FN2O_INLINE FV<N> vsqrt( FV<N> a )
+    {
+        FN2O_CF( 1 );
+#if defined( __AVX512F__ )
+        if constexpr( N == 16 ) return mkf<N>( (typename FV<N>::T)_mm512_sqrt_ps( (__m512)a.v ) );
+#endif
+        FV<N> r;
+        FN2O_LOOP r.v[k] = sqrtf( a.v[k] );
+        return r;
+    }
+    // x86 minps / maxps operand order: the second operand when unordered
+    template<int N> FN2O_INLINE FV<N> vmin( FV<N> a, FV<N> b ) { FN2O_CV(); return mkf<N>( a.v < b.v ? a.v : b.v ); }
+    template<int N> FN2O_INLINE FV<N> vmax( FV<N> a, FV<N> b ) { FN2O_CV(); return mkf<N>( a.v > b.v ? a.v : b.v ); }
     // FS::InvSqrt and FS::Reciprocal under strict semantics: correctly rounded 1/sqrt(x) and 1/x.
-    template<int N> inline FV<N> vinvsqrt( FV<N> a ) { FN2O_CF( 2 ); FV<N> r; FN2O_LOOP r.v[k] = 1.0f / sqrtf( a.v[k] ); return r; }
-    template<int N> inline FV<N> vrecip( FV<N> a ) { FN2O_CF( 1 ); FV<N> r; FN2O_LOOP r.v[k] = 1.0f / a.v[k]; return r; }
+    template<int N> FN2O_INLINE FV<N> vinvsqrt( FV<N> a )
+    {
+        FN2O_CF( 2 );
+        FV<N> s;
+#if defined( __AVX512F__ )
+        if constexpr( N == 16 ) s = mkf<N>( (typename FV<N>::T)_mm512_sqrt_ps( (__m512)a.v ) );
+        else
+#endif
+        {
+            FN2O_LOOP s.v[k] = sqrtf( a.v[k] );
+        }
+        return mkf<N>( 1.0f / s.v );
+    }
+    template<int N> FN2O_INLINE FV<N> vrecip( FV<N> a ) { FN2O_CF( 1 ); return mkf<N>( 1.0f / a.v ); }
 
     // ---- bit casts and conversions
-    template<int N> inline IV<N> fbits( FV<N> a ) { IV<N> r; FN2O_LOOP r.v[k] = (int32_t)bits_of( a.v[k] ); return r; }
-    template<int N> inline FV<N> ibits( IV<N> a ) { FV<N> r; FN2O_LOOP r.v[k] = float_of( (uint32_t)a.v[k] ); return r; }
-    template<int N> inline FV<N> cvtf( IV<N> a ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = (float)a.v[k]; return r; }
+    template<int N> FN2O_INLINE IV<N> fbits( FV<N> a ) { return mki<N>( (typename IV<N>::T)a.v ); }
+    template<int N> FN2O_INLINE FV<N> ibits( IV<N> a ) { return mkf<N>( (typename FV<N>::T)a.v ); }
+    template<int N> FN2O_INLINE FV<N> cvtf( IV<N> a ) { FN2O_CV(); return mkf<N>( __builtin_convertvector( a.v, typename FV<N>::T ) ); }
     static inline int32_t cvti1( float f )
     {
         // x86 cvtps2dq: round to nearest even, INT_MIN when out of range or NaN
         if( !( f >= -2147483648.0f && f < 2147483648.0f ) ) return INT32_MIN;
         return (int32_t)rintf( f );
     }
-    template<int N> inline IV<N> cvti( FV<N> a ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = cvti1( a.v[k] ); return r; }
+    template<int N> FN2O_INLINE IV<N> cvti( FV<N> a )
+    {
+        FN2O_CV();
+#if defined( __AVX512F__ )
+        if constexpr( N == 16 ) return mki<N>( (typename IV<N>::T)_mm512_cvtps_epi32( (__m512)a.v ) );
+#endif
+        IV<N> r;
+        FN2O_LOOP r.v[k] = cvti1( a.v[k] );
+        return r;
+    }
     // float xor float (sign manipulation), float | float
-    template<int N> inline FV<N> operator^( FV<N> a, FV<N> b ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = float_of( bits_of( a.v[k] ) ^ bits_of( b.v[k] ) ); return r; }
-    template<int N> inline FV<N> operator|( FV<N> a, FV<N> b ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = float_of( bits_of( a.v[k] ) | bits_of( b.v[k] ) ); return r; }
+    template<int N> FN2O_INLINE FV<N> operator^( FV<N> a, FV<N> b ) { FN2O_CV(); return ibits( mki<N>( fbits( a ).v ^ fbits( b ).v ) ); }
+    template<int N> FN2O_INLINE FV<N> operator|( FV<N> a, FV<N> b ) { FN2O_CV(); return ibits( mki<N>( fbits( a ).v | fbits( b ).v ) ); }
 
-    // ---- integer arithmetic (two's complement wrapping)
-    template<int N> inline IV<N> operator+( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = (int32_t)( (uint32_t)a.v[k] + (uint32_t)b.v[k] ); return r; }
-    template<int N> inline IV<N> operator-( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = (int32_t)( (uint32_t)a.v[k] - (uint32_t)b.v[k] ); return r; }
-    template<int N> inline IV<N> operator*( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = (int32_t)( (uint32_t)a.v[k] * (uint32_t)b.v[k] ); return r; }
-    template<int N> inline IV<N> operator^( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] ^ b.v[k]; return r; }
-    template<int N> inline IV<N> operator&( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] & b.v[k]; return r; }
-    template<int N> inline IV<N> operator|( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] | b.v[k]; return r; }
-    template<int N> inline IV<N> operator~( IV<N> a ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = ~a.v[k]; return r; }
-    template<int N> inline IV<N> operator+( IV<N> a, int32_t b ) { return a + isplat<N>( b ); }
-    template<int N> inline IV<N> operator-( IV<N> a, int32_t b ) { return a - isplat<N>( b ); }
-    template<int N> inline IV<N> operator*( IV<N> a, int32_t b ) { return a * isplat<N>( b ); }
-    template<int N> inline IV<N> operator&( IV<N> a, int32_t b ) { return a & isplat<N>( b ); }
-    template<int N> inline IV<N> operator^( IV<N> a, int32_t b ) { return a ^ isplat<N>( b ); }
-    template<int N> inline IV<N> operator|( IV<N> a, int32_t b ) { return a | isplat<N>( b ); }
-    template<int N> inline IV<N>& operator+=( IV<N>& a, IV<N> b ) { a = a + b; return a; }
-    template<int N> inline IV<N>& operator*=( IV<N>& a, IV<N> b ) { a = a * b; return a; }
-    template<int N> inline IV<N>& operator^=( IV<N>& a, IV<N> b ) { a = a ^ b; return a; }
+    // ---- integer arithmetic (two's complement wrapping: done on unsigned lanes)
+    template<int N> FN2O_INLINE IV<N> operator+( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( uv( a ) + uv( b ) ) ); }
+    template<int N> FN2O_INLINE IV<N> operator-( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( uv( a ) - uv( b ) ) ); }
+    template<int N> FN2O_INLINE IV<N> operator*( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( uv( a ) * uv( b ) ) ); }
+    template<int N> FN2O_INLINE IV<N> operator^( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v ^ b.v ); }
+    template<int N> FN2O_INLINE IV<N> operator&( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v & b.v ); }
+    template<int N> FN2O_INLINE IV<N> operator|( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v | b.v ); }
+    template<int N> FN2O_INLINE IV<N> operator~( IV<N> a ) { FN2O_CV(); return mki<N>( ~a.v ); }
+    template<int N> FN2O_INLINE IV<N> operator+( IV<N> a, int32_t b ) { return a + isplat<N>( b ); }
+    template<int N> FN2O_INLINE IV<N> operator-( IV<N> a, int32_t b ) { return a - isplat<N>( b ); }
+    template<int N> FN2O_INLINE IV<N> operator*( IV<N> a, int32_t b ) { return a * isplat<N>( b ); }
+    template<int N> FN2O_INLINE IV<N> operator&( IV<N> a, int32_t b ) { return a & isplat<N>( b ); }
+    template<int N> FN2O_INLINE IV<N> operator^( IV<N> a, int32_t b ) { return a ^ isplat<N>( b ); }
+    template<int N> FN2O_INLINE IV<N> operator|( IV<N> a, int32_t b ) { return a | isplat<N>( b ); }
+    template<int N> FN2O_INLINE IV<N>& operator+=( IV<N>& a, IV<N> b ) { a = a + b; return a; }
+    template<int N> FN2O_INLINE IV<N>& operator*=( IV<N>& a, IV<N> b ) { a = a * b; return a; }
+    template<int N> FN2O_INLINE IV<N>& operator^=( IV<N>& a, IV<N> b ) { a = a ^ b; return a; }
     // arithmetic shift right, shift left, logical shift right
-    template<int N> inline IV<N> sra( IV<N> a, int s ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] >> s; return r; }
-    template<int N> inline IV<N> shl( IV<N> a, int s ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = (int32_t)( (uint32_t)a.v[k] << s ); return r; }
-    template<int N> inline IV<N> srl( IV<N> a, int s ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = (int32_t)( (uint32_t)a.v[k] >> s ); return r; }
+    template<int N> FN2O_INLINE IV<N> sra( IV<N> a, int s ) { FN2O_CV(); return mki<N>( a.v >> s ); }
+    template<int N> FN2O_INLINE IV<N> shl( IV<N> a, int s ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( uv( a ) << s ) ); }
+    template<int N> FN2O_INLINE IV<N> srl( IV<N> a, int s ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( uv( a ) >> s ) ); }
 
     // ---- masks: int32 lanes holding 0 or -1
-    template<int N> inline IV<N> mlt( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] < b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mgt( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] > b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mge( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] >= b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> meq( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] == b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mne( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] != b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mlt( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] < b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mgt( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] > b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mge( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] >= b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> meq( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] == b.v[k] ? -1 : 0; return r; }
-    template<int N> inline IV<N> mne( IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = a.v[k] != b.v[k] ? -1 : 0; return r; }
+    template<int N> FN2O_INLINE IV<N> mlt( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v < b.v ) ); }
+    template<int N> FN2O_INLINE IV<N> mgt( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v > b.v ) ); }
+    template<int N> FN2O_INLINE IV<N> mge( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v >= b.v ) ); }
+    template<int N> FN2O_INLINE IV<N> meq( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v == b.v ) ); }
+    template<int N> FN2O_INLINE IV<N> mne( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v != b.v ) ); }
+    template<int N> FN2O_INLINE IV<N> mlt( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v < b.v ); }
+    template<int N> FN2O_INLINE IV<N> mgt( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v > b.v ); }
+    template<int N> FN2O_INLINE IV<N> mge( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v >= b.v ); }
+    template<int N> FN2O_INLINE IV<N> meq( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v == b.v ); }
+    template<int N> FN2O_INLINE IV<N> mne( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v != b.v ); }
 
     // FS::Select( m, a, b ) = m ? a : b
-    template<int N> inline FV<N> sel( IV<N> m, FV<N> a, FV<N> b ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = m.v[k] ? a.v[k] : b.v[k]; return r; }
-    template<int N> inline IV<N> sel( IV<N> m, IV<N> a, IV<N> b ) { FN2O_CV(); IV<N> r; FN2O_LOOP r.v[k] = m.v[k] ? a.v[k] : b.v[k]; return r; }
+    template<int N> FN2O_INLINE FV<N> sel( IV<N> m, FV<N> a, FV<N> b ) { FN2O_CV(); return mkf<N>( m.v != 0 ? a.v : b.v ); }
+    template<int N> FN2O_INLINE IV<N> sel( IV<N> m, IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( m.v != 0 ? a.v : b.v ); }
     // FS::SelectHighBit( v, a, b ) = sign bit of v ? a : b
-    template<int N> inline FV<N> selhb( IV<N> m, FV<N> a, FV<N> b ) { FN2O_CV(); FV<N> r; FN2O_LOOP r.v[k] = m.v[k] < 0 ? a.v[k] : b.v[k]; return r; }
+    template<int N> FN2O_INLINE FV<N> selhb( IV<N> m, FV<N> a, FV<N> b ) { FN2O_CV(); return mkf<N>( m.v < 0 ? a.v : b.v ); }
     // Masked helpers (FS::Masked*, FS::InvMasked*)
-    template<int N> inline FV<N> masked( IV<N> m, FV<N> a ) { return sel( m, a, fsplat<N>( 0.0f ) ); }
-    template<int N> inline IV<N> masked( IV<N> m, IV<N> a ) { return sel( m, a, isplat<N>( 0 ) ); }
-    template<int N> inline IV<N> invmasked( IV<N> m, IV<N> a ) { return sel( m, isplat<N>( 0 ), a ); }
-    template<int N> inline FV<N> madd( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a + b, a ); }
-    template<int N> inline IV<N> madd( IV<N> m, IV<N> a, IV<N> b ) { return sel( m, a + b, a ); }
-    template<int N> inline FV<N> msub( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a - b, a ); }
-    template<int N> inline FV<N> mmul( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a * b, a ); }
-    template<int N> inline FV<N> imadd( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a, a + b ); }
-    template<int N> inline IV<N> imadd( IV<N> m, IV<N> a, IV<N> b ) { return sel( m, a, a + b ); }
-    template<int N> inline FV<N> imsub( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a, a - b ); }
-    template<int N> inline IV<N> imsub( IV<N> m, IV<N> a, IV<N> b ) { return sel( m, a, a - b ); }
-    template<int N> inline FV<N> minc( IV<N> m, FV<N> a ) { return sel( m, a + 1.0f, a ); }
-    template<int N> inline IV<N> minc( IV<N> m, IV<N> a ) { return sel( m, a + 1, a ); }
-    template<int N> inline FV<N> mdec( IV<N> m, FV<N> a ) { return sel( m, a - 1.0f, a ); }
-    template<int N> inline IV<N> mdec( IV<N> m, IV<N> a ) { return sel( m, a - 1, a ); }
-    template<int N> inline bool any( IV<N> m ) { bool r = false; FN2O_LOOP r |= m.v[k] != 0; return r; }
+    template<int N> FN2O_INLINE FV<N> masked( IV<N> m, FV<N> a ) { return sel( m, a, fsplat<N>( 0.0f ) ); }
+    template<int N> FN2O_INLINE IV<N> masked( IV<N> m, IV<N> a ) { return sel( m, a, isplat<N>( 0 ) ); }
+    template<int N> FN2O_INLINE IV<N> invmasked( IV<N> m, IV<N> a ) { return sel( m, isplat<N>( 0 ), a ); }
+    template<int N> FN2O_INLINE FV<N> madd( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a + b, a ); }
+    template<int N> FN2O_INLINE IV<N> madd( IV<N> m, IV<N> a, IV<N> b ) { return sel( m, a + b, a ); }
+    template<int N> FN2O_INLINE FV<N> msub( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a - b, a ); }
+    template<int N> FN2O_INLINE FV<N> mmul( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a * b, a ); }
+    template<int N> FN2O_INLINE FV<N> imadd( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a, a + b ); }
+    template<int N> FN2O_INLINE IV<N> imadd( IV<N> m, IV<N> a, IV<N> b ) { return sel( m, a, a + b ); }
+    template<int N> FN2O_INLINE FV<N> imsub( IV<N> m, FV<N> a, FV<N> b ) { return sel( m, a, a - b ); }
+    template<int N> FN2O_INLINE IV<N> imsub( IV<N> m, IV<N> a, IV<N> b ) { return sel( m, a, a - b ); }
+    template<int N> FN2O_INLINE FV<N> minc( IV<N> m, FV<N> a ) { return sel( m, a + 1.0f, a ); }
+    template<int N> FN2O_INLINE IV<N> minc( IV<N> m, IV<N> a ) { return sel( m, a + 1, a ); }
+    template<int N> FN2O_INLINE FV<N> mdec( IV<N> m, FV<N> a ) { return sel( m, a - 1.0f, a ); }
+    template<int N> FN2O_INLINE IV<N> mdec( IV<N> m, IV<N> a ) { return sel( m, a - 1, a ); }
+    template<int N> FN2O_INLINE bool any( IV<N> m ) { bool r = false; FN2O_LOOP r |= m.v[k] != 0; return r; }
 }

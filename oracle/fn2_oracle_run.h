@@ -19,12 +19,25 @@ namespace FN2O_NS
         template<int D>
         static void block( const DriverArgs& a, int64_t begin, int64_t end, MinMax& mm )
         {
+            // per-axis index of the current point, x innermost (Generator.inl:447-456):
+            // decomposed once per slab, then advanced by carries like AxisReset
+            int32_t axis[4] = { 0, 0, 0, 0 };
+            if( a.kind != 1 )
+            {
+                int64_t rem = begin;
+                for( int d = 0; d < D; d++ )
+                {
+                    axis[d] = (int32_t)( rem % a.count[d] );
+                    rem /= a.count[d];
+                }
+            }
             for( int64_t base = begin; base < end; base += N )
             {
                 Pos<D, N> p;
                 for( int k = 0; k < N; k++ )
                 {
-                    int64_t idx = base + k < end ? base + k : end - 1;
+                    const bool live = base + k < end;
+                    int64_t idx = live ? base + k : end - 1;
                     if( a.kind == 1 )
                     {
                         // GenPositionArray: offset + array value (Generator.inl:265-266, 295-297, 327-330)
@@ -32,13 +45,14 @@ namespace FN2O_NS
                     }
                     else
                     {
-                        // index -> per-axis index, x innermost (Generator.inl:447-456)
-                        int64_t rem = idx;
-                        for( int d = 0; d < D; d++ )
+                        for( int d = 0; d < D; d++ ) p.c[d].v[k] = (float)axis[d];
+                        if( live && base + k + 1 < end )
                         {
-                            int32_t ai = (int32_t)( rem % a.count[d] );
-                            rem /= a.count[d];
-                            p.c[d].v[k] = (float)ai;
+                            for( int d = 0; d < D; d++ )
+                            {
+                                if( ++axis[d] < a.count[d] || d == D - 1 ) break;
+                                axis[d] = 0;
+                            }
                         }
                     }
                 }

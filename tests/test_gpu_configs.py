@@ -128,3 +128,18 @@ def test_gen_single_matches_grid(gpu):
         v = node.gen_single(x, y, seed=42)
         r = t.single(x, y, seed=42)
         assert np.float32(v) == np.float32(r) or (np.isnan(v) and np.isnan(r))
+
+
+@pytest.mark.parametrize("dims", [2, 3, 4])
+def test_mixed_serving_trees(gpu, dims):
+    # the 8 seeded random trees of the mixed serving workload (depth 3..12)
+    for enc in W.mixed_trees():
+        node = fn.Node.from_encoded(enc)
+        t = oracle.Tree(enc)
+        off = [-1234.5, 777.25, 31.0, -8.5][:dims]
+        cnt = [24, 20, 12, 6][:dims]
+        gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d, 4: node.gen_uniform_grid_4d}[dims]
+        out, mm = gen(*off, *cnt, seed=99)
+        ref, ref_mm = t.grid(off, cnt, seed=99)
+        assert_bitexact(out, ref, "mixed tree %s %dD" % (enc[:12], dims))
+        assert mm == ref_mm

@@ -79,6 +79,10 @@ def test_alone_equals_in_mix(gpu, mode):
     st = eng.stats()
     assert st["completed"] == len(reqs)
     if mode != SERVE_PLAIN:
+        for t in tids:
+            assert all(eng.tree_tuning(t, d) in (0, 1, 2, 4, 8) for d in (2, 3, 4))
+        assert any(eng.tree_tuning(t, 3) in (1, 2, 4, 8) for t in tids)
+    if mode != SERVE_PLAIN:
         assert st["batches"] < len(reqs)
     for (t, spec), (out, mm, lat) in zip(reqs, results):
         ref, ref_mm = alone(nodes, t, spec)
