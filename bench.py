@@ -41,6 +41,10 @@ def workloads(W):
                                                     count=[256, 256, 256], seed=1337),
         "warped_cellular_positions_16777216": dict(tree=W.warped_cellular_3d, kind="positions", dims=3, count=[1 << 24],
                                                    seed=1337),
+        "serving_tree_3d_grid_256x256x256": dict(tree=W.serving_tree, kind="grid", dims=3, offset=[0, 0, 0],
+                                                 count=[256, 256, 256], seed=1337),
+        "mixed_tree_deepest_3d_grid_128x128x128": dict(tree=lambda: W.decode(W.mixed_trees()[-1]), kind="grid", dims=3,
+                                                       offset=[0, 0, 0], count=[128, 128, 128], seed=1337),
     }
 
 

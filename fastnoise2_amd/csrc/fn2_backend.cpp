@@ -296,9 +296,10 @@ namespace fn2
         for( int d = 0; d < axes; d++ )
         {
             if( s.count[d] < 0 ) return 0;
-            n *= s.count[d];
+            n *= s.count[d];   // both factors < 2^31: no overflow
+            if( n > 0x7fffffff ) return 0;
         }
-        return n > 0x7fffffff ? 0 : n;
+        return n;
     }
 
     // GenTileable2D maps column x and row y onto a 4D torus (Generator.inl:370-433
@@ -363,6 +364,7 @@ namespace fn2
             a.dims = 4;
             if( s.count[0] < 0 || s.count[1] < 0 ) { err = "negative size"; return false; }
             total = (int64_t)s.count[0] * s.count[1];
+            if( total > 0x7fffffff ) { err = "more than 2^31-1 points in one call"; return false; }
             if( total > 0 )
             {
                 a.div[0] = MakeFastDiv( (uint32_t)s.count[0] );
@@ -381,7 +383,8 @@ namespace fn2
             for( int d = 0; d < s.dims; d++ )
             {
                 if( s.count[d] < 0 ) { err = "negative count"; return false; }
-                total *= s.count[d];
+                total *= s.count[d];   // both factors < 2^31: no overflow
+                if( total > 0x7fffffff ) { err = "more than 2^31-1 points in one call"; return false; }
                 a.offset[d] = s.offset[d];
                 a.step[d] = s.step[d];
             }
@@ -423,19 +426,20 @@ namespace fn2
         GenArgs a{};
         int64_t total = 0;
         std::string err;
+        // points the caller's buffer holds: 0 when the spec is invalid or too large
+        const int64_t fillCount = SpecPoints( spec );
         auto fail = [&]( const std::string& msg ) -> OutputRange
         {
             SetError( msg );
-            if( out && total > 0 && !IsDevicePointer( out ) )
+            if( out && fillCount > 0 && !IsDevicePointer( out ) )
             {
-                for( int64_t i = 0; i < total; i++ ) out[i] = __builtin_nanf( "" );
+                for( int64_t i = 0; i < fillCount; i++ ) out[i] = __builtin_nanf( "" );
             }
             OutputRange bad;
             bad.min = bad.max = __builtin_nanf( "" );
             bad.ok = false;
             return bad;
         };
-        total = SpecPoints( spec );
         if( !EnsureDevice() ) return fail( gInitError );
         if( !FillSpecArgs( spec, a, total, err ) ) return fail( err );
         if( total == 0 ) return r;

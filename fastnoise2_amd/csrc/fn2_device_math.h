@@ -52,10 +52,25 @@ namespace fn2d
     FN2D int32_t fbits( float a ) { return __builtin_bit_cast( int32_t, a ); }
     FN2D float ibits( int32_t a ) { return __builtin_bit_cast( float, a ); }
     FN2D float fxor( float a, int32_t signBits ) { return ibits( fbits( a ) ^ signBits ); }
+    // FS::Convert<int32_t> of an integral float with the x86 cvtps2dq result
+    // for NaN and out-of-range inputs: INT_MIN. v_cvt_i32_f32 saturates
+    // instead (NaN -> 0, >= 2^31 -> INT_MAX, < -2^31 -> INT_MIN), so only NaN
+    // and a >= 2^31 need the select. The hardware instruction is issued
+    // directly because an out-of-range C++ float->int conversion is undefined.
+    FN2D int32_t cvtIntegral( float a )
+    {
+#if defined( __HIP_DEVICE_COMPILE__ )
+        int32_t v;
+        asm volatile( "v_cvt_i32_f32 %0, %1" : "=v"( v ) : "v"( a ) );
+#else
+        int32_t v = a >= -2147483648.0f && a < 2147483648.0f ? (int32_t)a : (int32_t)0x80000000u;
+#endif
+        return a < 2147483648.0f ? v : (int32_t)0x80000000u;
+    }
     // floor() results are integral, so the truncating convert is exact
-    FN2D int32_t floorToInt( float a ) { return (int32_t)a; }
+    FN2D int32_t floorToInt( float a ) { return cvtIntegral( a ); }
     // FS::Convert<int32_t> of a non-integer: round to nearest even
-    FN2D int32_t roundToInt( float a ) { return (int32_t)__builtin_rintf( a ); }
+    FN2D int32_t roundToInt( float a ) { return cvtIntegral( __builtin_rintf( a ) ); }
     FN2D float invSqrt( float a ) { return 1.0f / __builtin_sqrtf( a ); }
 
     // Correctly rounded sqrt and reciprocal for positive, normal, finite

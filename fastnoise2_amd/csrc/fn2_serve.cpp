@@ -114,6 +114,96 @@ namespace
 
     size_t AlignUp( size_t v, size_t a ) { return ( v + a - 1 ) / a * a; }
 
+    // Host-side copies of a retired batch (pinned staging -> callers' buffers)
+    // spread over a few threads: one thread's memcpy bandwidth would otherwise
+    // bound the engine at roughly 10 GB/s of output.
+    class CopyPool
+    {
+    public:
+        struct Seg { float* dst; const float* src; size_t n; };
+
+        explicit CopyPool( int threads )
+        {
+            for( int t = 0; t < threads; t++ ) workers.emplace_back( [this, t] { Work( t ); } );
+        }
+        ~CopyPool()
+        {
+            {
+                std::lock_guard<std::mutex> lk( m );
+                quit = true;
+                gen++;
+            }
+            cv.notify_all();
+            for( auto& w : workers ) w.join();
+        }
+        // copies every segment; returns when all are done
+        void Run( const std::vector<Seg>& segs )
+        {
+            size_t total = 0;
+            for( const Seg& g : segs ) total += g.n;
+            if( total < ( 1u << 18 ) || workers.empty() )
+            {
+                for( const Seg& g : segs ) memcpy( g.dst, g.src, g.n * 4 );
+                return;
+            }
+            // split into equal float ranges, one per worker plus the caller
+            parts.assign( workers.size() + 1, {} );
+            size_t per = ( total + parts.size() - 1 ) / parts.size();
+            size_t p = 0, fill = 0;
+            for( const Seg& g : segs )
+            {
+                size_t off = 0;
+                while( off < g.n )
+                {
+                    // the last part takes whatever remains
+                    size_t take = p + 1 < parts.size() ? std::min( g.n - off, per - fill ) : g.n - off;
+                    parts[p].push_back( { g.dst + off, g.src + off, take } );
+                    off += take;
+                    fill += take;
+                    if( fill == per && p + 1 < parts.size() ) { p++; fill = 0; }
+                }
+            }
+            {
+                std::lock_guard<std::mutex> lk( m );
+                pending = (int)workers.size();
+                gen++;
+            }
+            cv.notify_all();
+            for( const Seg& g : parts.back() ) memcpy( g.dst, g.src, g.n * 4 );
+            std::unique_lock<std::mutex> lk( m );
+            cvDone.wait( lk, [&] { return pending == 0; } );
+        }
+
+    private:
+        void Work( int t )
+        {
+            uint64_t seen = 0;
+            for( ;; )
+            {
+                {
+                    std::unique_lock<std::mutex> lk( m );
+                    cv.wait( lk, [&] { return gen != seen; } );
+                    seen = gen;
+                    if( quit ) return;
+                }
+                for( const Seg& g : parts[t] ) memcpy( g.dst, g.src, g.n * 4 );
+                {
+                    std::lock_guard<std::mutex> lk( m );
+                    pending--;
+                }
+                cvDone.notify_all();
+            }
+        }
+
+        std::vector<std::thread> workers;
+        std::vector<std::vector<Seg>> parts;
+        std::mutex m;
+        std::condition_variable cv, cvDone;
+        uint64_t gen = 0;
+        int pending = 0;
+        bool quit = false;
+    };
+
     class Engine
     {
     public:
@@ -176,6 +266,7 @@ namespace
                 }
                 deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + 8 + tb * 8 );
             }
+            copies.reset( new CopyPool( 3 ) );
             dispatcher = std::thread( [this] { DispatchLoop(); } );
             completer = std::thread( [this] { CompleteLoop(); } );
             return true;
@@ -449,6 +540,7 @@ namespace
         int maxBlocks = 0;
         int nextSlot = 0;
         std::thread dispatcher, completer;
+        std::unique_ptr<CopyPool> copies;
 
         // Fills slot s with queued requests (FIFO) up to its capacity; the plain
         // mode takes one request per launch. Called with mu held.
@@ -608,9 +700,12 @@ namespace
                 bool ok = code >= 0 && fn2::HipOk( hipEventSynchronize( s.done ), "serve batch" );
                 if( ok )
                 {
+                    std::vector<CopyPool::Seg> segs;
+                    segs.reserve( s.reqs.size() );
+                    for( Request* r : s.reqs ) segs.push_back( { r->hostOut, s.hOut + r->outOffset, (size_t)r->points } );
+                    copies->Run( segs );
                     for( Request* r : s.reqs )
                     {
-                        memcpy( r->hostOut, s.hOut + r->outOffset, (size_t)r->points * 4 );
                         int32_t k0 = s.hKeys[2 * r->job], k1 = s.hKeys[2 * r->job + 1];
                         if( k0 == 0x7fffffff && k1 == (int32_t)0x80000000u )
                         {

@@ -6,7 +6,7 @@ reference: mFrequency = 1 / scale), so 0.02 -> 50, 0.005 -> 200, 0.01 -> 100.
 """
 import numpy as np
 
-from .tree import NodeSpec
+from .tree import NodeSpec, decode
 
 S = NodeSpec
 

@@ -1,10 +1,11 @@
 // Test infrastructure only: the lane types of the CPU oracle.
 //
 // The oracle restates FastNoise2's generator path on the CPU. It evaluates
-// N points at once through fixed-width "lane" arrays so that the same source
+// N points at once through fixed-width "lane" vectors so that the same source
 // serves two purposes: with N = 1 it is the scalar checker used by the
-// parity tests, and with N = 16 (compiled with AVX2/AVX-512 flags) gcc turns
-// every per-lane loop into vector instructions, which gives the host SIMD
+// parity tests, and with N = 16 (compiled with AVX2/AVX-512 flags) every
+// lane operation is a vector instruction (GCC vector types, x86 intrinsics
+// for fma / floor / rint / sqrt / float->int), which gives the host SIMD
 // baseline timed by bench.py. Only tests/, __graft_entry__.smoke() and the
 // cpu_baseline leg of bench.py may load the oracle; nothing in the product
 // links it.

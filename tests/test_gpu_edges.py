@@ -1,0 +1,129 @@
+"""Edge cases of the batch entry points on the GPU, through the C ABI:
+empty and single-point calls, ragged sizes that do not fill a wave or a
+block, NaN / infinite / huge positions, extreme seeds, and output buffers
+that are device pointers. Each result is compared with the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import fastnoise2_amd as fn
+from fastnoise2_amd import _lib
+from fastnoise2_amd.tree import NodeSpec as S
+import oracle
+from conftest import assert_bitexact
+
+pytestmark = pytest.mark.gpu
+
+TREE = S("FractalFBm", source=S("CellularDistance", feature_scale=13.0, distance_function="Euclidean"), octaves=2)
+
+
+def test_empty_calls_write_nothing(gpu):
+    node = fn.Node.from_encoded(TREE.encode())
+    L = _lib.load()
+    sentinel = np.full(4, 7.0, dtype=np.float32)
+    mm = np.zeros(2, dtype=np.float32)
+    fp = _lib.c_float_p
+    L.fnGenUniformGrid2D(node.handle, sentinel.ctypes.data_as(fp), 0, 0, 0, 5, 1, 1, 1, mm.ctypes.data_as(fp))
+    assert (sentinel == 7.0).all() and mm[0] == np.inf and mm[1] == -np.inf
+    L.fnGenUniformGrid3D(node.handle, sentinel.ctypes.data_as(fp), 0, 0, 0, 3, 0, 3, 1, 1, 1, 1, mm.ctypes.data_as(fp))
+    assert (sentinel == 7.0).all() and mm[0] == np.inf
+    L.fnGenPositionArray2D(node.handle, sentinel.ctypes.data_as(fp), 0, sentinel.ctypes.data_as(fp),
+                           sentinel.ctypes.data_as(fp), 0, 0, 1, mm.ctypes.data_as(fp))
+    assert (sentinel == 7.0).all() and mm[0] == np.inf
+    L.fnGenTileable2D(node.handle, sentinel.ctypes.data_as(fp), 0, 0, 1, 1, 1, mm.ctypes.data_as(fp))
+    assert (sentinel == 7.0).all() and mm[0] == np.inf
+    assert fn.last_error() == ""
+
+
+@pytest.mark.parametrize("count", [[1, 1], [1, 63], [65, 1], [3, 257], [255, 3], [17, 31]])
+def test_ragged_2d_grids(gpu, count):
+    node = fn.Node.from_encoded(TREE.encode())
+    t = oracle.Tree(TREE.encode())
+    out, mm = node.gen_uniform_grid_2d(-3.5, 11.25, *count, 0.75, 1.25, seed=7)
+    ref, ref_mm = t.grid([-3.5, 11.25], count, [0.75, 1.25], seed=7)
+    assert_bitexact(out, ref, "ragged %r" % (count,))
+    assert mm == ref_mm
+
+
+@pytest.mark.parametrize("count", [[1, 1, 1], [7, 1, 33], [2, 129, 2], [5, 5, 5, 5], [1, 2, 3, 4]])
+def test_ragged_3d_4d_grids(gpu, count):
+    node = fn.Node.from_encoded(TREE.encode())
+    t = oracle.Tree(TREE.encode())
+    d = len(count)
+    off = [1.5, -2.25, 100.0, -7.0][:d]
+    gen = {3: node.gen_uniform_grid_3d, 4: node.gen_uniform_grid_4d}[d]
+    out, mm = gen(*off, *count, seed=-9)
+    ref, ref_mm = t.grid(off, count, seed=-9)
+    assert_bitexact(out, ref, "ragged %r" % (count,))
+    assert mm == ref_mm
+
+
+def test_special_positions(gpu):
+    node = fn.Node.from_encoded(TREE.encode())
+    t = oracle.Tree(TREE.encode())
+    xs = np.array([0.0, -0.0, 1e-30, -1e-30, 1e6, -1e6, 3.0e7, np.inf, -np.inf, np.nan, 2.5, 1e38], dtype=np.float32)
+    ys = np.array([0.0, 0.0, 1.0, -1.0, 1e6, 5.5, -3.0e7, 0.0, 1.0, 2.0, np.nan, -1e38], dtype=np.float32)
+    out, mm = node.gen_position_array([xs, ys], seed=3)
+    ref, ref_mm = t.positions([xs, ys], seed=3)
+    assert_bitexact(out, ref, "special positions")
+    assert mm == ref_mm or (np.isnan(mm[0]) and np.isnan(ref_mm[0]))
+
+
+@pytest.mark.parametrize("seed", [0, 1, -1, 2**31 - 1, -2**31])
+def test_extreme_seeds(gpu, seed):
+    for spec in (TREE, S("DomainWarpSimplex", source=S("Value"), warp_amplitude=20.0, seed_offset=-5)):
+        node = fn.Node.from_encoded(spec.encode())
+        out, _ = node.gen_uniform_grid_3d(-10, 20, 30, 9, 8, 7, seed=seed)
+        ref, _ = oracle.Tree(spec.encode()).grid([-10, 20, 30], [9, 8, 7], seed=seed)
+        assert_bitexact(out, ref, "seed %d" % seed)
+
+
+def test_device_output_pointer_through_c_abi(gpu):
+    node = fn.Node.from_encoded(TREE.encode())
+    n = 37 * 41
+    d = fn.DeviceArray(n)
+    mm = np.zeros(2, dtype=np.float32)
+    _lib.load().fnGenUniformGrid2D(node.handle, ctypes.cast(d.ptr, _lib.c_float_p), -5, 5, 37, 41, 1, 1, 11,
+                                   mm.ctypes.data_as(_lib.c_float_p))
+    ref, ref_mm = oracle.Tree(TREE.encode()).grid([-5, 5], [37, 41], seed=11)
+    assert_bitexact(d.to_host(), ref, "device output")
+    assert tuple(mm) == ref_mm
+
+
+def test_concurrent_host_calls(gpu):
+    import threading
+    node = fn.Node.from_encoded(TREE.encode())
+    ref, _ = oracle.Tree(TREE.encode()).grid([0, 0], [96, 80], seed=5)
+    errors = []
+
+    def worker():
+        for _ in range(5):
+            out, _ = node.gen_uniform_grid_2d(0, 0, 96, 80, seed=5)
+            if not np.array_equal(out, ref):
+                errors.append("mismatch")
+
+    ths = [threading.Thread(target=worker) for _ in range(6)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert not errors
+
+
+def test_oversized_request_is_rejected(gpu):
+    # 2^32 points: rejected before anything is written (the reference's int
+    # product would overflow); the small sentinel buffer stays untouched
+    node = fn.Node.from_encoded(TREE.encode())
+    L = _lib.load()
+    sentinel = np.full(4, 7.0, dtype=np.float32)
+    mm = np.zeros(2, dtype=np.float32)
+    fp = _lib.c_float_p
+    L.fnGenUniformGrid3D(node.handle, sentinel.ctypes.data_as(fp), 0, 0, 0, 2048, 2048, 1024, 1, 1, 1, 1,
+                         mm.ctypes.data_as(fp))
+    assert (sentinel == 7.0).all()
+    assert np.isnan(mm).all()
+    assert "2^31" in fn.last_error()
+    L.fnGenTileable2D(node.handle, sentinel.ctypes.data_as(fp), 1 << 30, 1 << 30, 1, 1, 1, mm.ctypes.data_as(fp))
+    assert (sentinel == 7.0).all()
+    assert "2^31" in fn.last_error()
