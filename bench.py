@@ -100,7 +100,7 @@ def cpu_baseline(wl, enc, target_s=12.0):
 def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
     """Chunk latency and throughput of the serving engine on the serving tree:
     the Poisson trace (open loop) for p50/p99 latency, and a saturating closed
-    loop (128 outstanding) of the same request mix for throughput, in the
+    loop (512 outstanding) of the same request mix for throughput, in the
     plain (one launch + copy per chunk), batched and graph-replay modes."""
     from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
     enc = W.serving_tree().encode()
@@ -118,7 +118,7 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
             rec.update({"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                         "mean_ms": round(float(lat.mean()), 3), "trace_wall_s": round(wall, 2)})
         n = min(saturate_requests, len(specs))
-        lat, wall = eng.replay([tid] * n, specs[:n], None, 128)
+        lat, wall = eng.replay([tid] * n, specs[:n], None, 512)
         rec.update({"saturated_Gpoints_s": round(float(pts[:n].sum()) / wall / 1e9, 3),
                     "saturated_requests_s": round(n / wall, 1),
                     "saturated_p50_ms": round(float(np.percentile(lat, 50)), 3),
@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--jit", default="sync", choices=["off", "sync"],
+                    help="run-time specialised kernel of the tree (compiled before timing) or the interpreter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -202,6 +204,17 @@ def main():
     n = int(np.prod(wl["count"]))
     out = fn.DeviceArray(n)
 
+    # the interpreter's figure first, then the tree's specialised kernel
+    fn.set_jit_mode(fn.JIT_OFF)
+    interp_ms, _ = node.bench_device(spec, out, warmup=args.warmup, iters=max(3, args.steps // 5))
+    kernel_name = "fn2_generate"
+    if args.jit == "sync":
+        fn.set_jit_mode(fn.JIT_SYNC)
+        t0 = time.perf_counter()
+        node.bench_device(spec, out, warmup=0, iters=1)
+        jit_compile_s = time.perf_counter() - t0
+        if node.jit_state(wl["dims"] if wl["kind"] != "tileable" else 4) == 1:
+            kernel_name = "fn2_jit_single"
     ms_event, ms_wall = node.bench_device(spec, out, warmup=args.warmup, iters=args.steps)
     value = n / (ms_wall * 1e-3) / 1e9
 
@@ -225,7 +238,8 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
         "traffic": None,
-        "kernel": "fn2_generate",
+        "kernel": kernel_name,
+        "interpreter_kernel_ms": round(interp_ms, 4),
         "kernel_ms": round(ms_event, 4),
         "flop_per_point": flop_pt,
         "valu_ops_per_point": valu_pt,
@@ -248,6 +262,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: grid coordinates / SplitMix64(7) positions, output device-resident",
         "config": {"workload": args.workload, "tree": enc, "points": n, "seed": wl["seed"]},
+        "jit_compile_s": round(jit_compile_s, 2) if args.jit == "sync" else None,
         "roofline": roofline,
     }
     if wl["kind"] == "grid":

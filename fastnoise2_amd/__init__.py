@@ -30,6 +30,20 @@ def device_count():
     return lib().fn2DeviceCount()
 
 
+JIT_OFF, JIT_ASYNC, JIT_SYNC = 0, 1, 2
+
+
+def set_jit_mode(mode):
+    """Run-time specialisation of trees: JIT_OFF, JIT_ASYNC (default) or JIT_SYNC.
+    Returns the previous mode."""
+    return lib().fn2SetJitMode(mode)
+
+
+def jit_wait():
+    """Waits for every queued specialisation to finish compiling."""
+    lib().fn2JitWait()
+
+
 def _fp(a):
     return a.ctypes.data_as(_lib.c_float_p)
 
@@ -230,6 +244,10 @@ class Node:
         if rc:
             raise RuntimeError("fn2BenchGenerate failed: " + last_error())
         return ms.value, wall.value
+
+    def jit_state(self, dims):
+        """-1 never requested, 0 queued, 1 ready, 2 failed."""
+        return lib().fn2GetJitState(self.handle, dims)
 
     def program_info(self, dims):
         ins, regs, dm, fm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -99,6 +99,10 @@ GPU_EXT = {
     "fn2DeviceCount": (i, []),
     "fn2GetLastError": (s, []),
     "fn2SelfTestMath": (i, [i, vp, vp, i]),
+    "fn2SetJitMode": (i, [i]),
+    "fn2JitWait": (i, []),
+    "fn2JitCompileCheck": (i, [vp, i, c_int_p]),
+    "fn2GetJitState": (i, [vp, i]),
     "fn2GetProgramInfo": (i, [vp, i, c_int_p, c_int_p, c_int_p, c_int_p]),
     "fn2ServeCreate": (vp, [ctypes.c_size_t, i, i]),
     "fn2ServeDestroy": (None, [vp]),

@@ -24,10 +24,12 @@
 
 #include "fn2_backend.h"
 #include "fn2_internal.h"
+#include "fn2_jit.h"
 #include "fn2_gpu.h"
 
 extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
+unsigned fn2_grid_for( int64_t total );
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
 int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStream_t stream );
 }
@@ -121,6 +123,20 @@ namespace fn2
         hipError_t e = hipMemcpyAsync( dst, src, bytes, hipMemcpyHostToDevice, gUploadStream );
         if( e == hipSuccess ) e = hipStreamSynchronize( gUploadStream );
         return e;
+    }
+
+    hipError_t LaunchProgram( const GenArgs& a, const DeviceProgram* dp, hipStream_t stream, bool* specialised )
+    {
+        if( a.total <= 0 ) return hipSuccess;
+        if( const JitKernel* k = GetJitKernel( dp ) )
+        {
+            if( specialised ) *specialised = true;
+            GenArgs copy = a;
+            void* params[] = { &copy };
+            return hipModuleLaunchKernel( k->single, fn2_grid_for( a.total ), 1, 1, 256, 1, 1, 0, stream, params, nullptr );
+        }
+        if( specialised ) *specialised = false;
+        return (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, stream );
     }
 
     FastDiv MakeFastDiv( uint32_t d )
@@ -487,7 +503,7 @@ namespace fn2
         }
 
         if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, c->stream ), "launch min/max init" ) ) return fail( tLastError );
-        if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, c->stream ), "launch generator" ) ) return fail( tLastError );
+        if( !HipOk( LaunchProgram( a, dp, c->stream, nullptr ), "launch generator" ) ) return fail( tLastError );
         if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, 2 * sizeof( int32_t ), hipMemcpyDeviceToHost, c->stream ), "read min/max" ) ) return fail( tLastError );
         if( !devOut )
         {
@@ -717,7 +733,7 @@ FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, f
     a.out = dOut;
     a.minmax = outputMinMax ? c->dKeys : nullptr;
     if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, st ), "launch min/max init" ) ) return 1;
-    if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, st ), "launch generator" ) ) return 1;
+    if( !HipOk( LaunchProgram( a, dp, st, nullptr ), "launch generator" ) ) return 1;
     if( outputMinMax )
     {
         if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, 2 * sizeof( int32_t ), hipMemcpyDeviceToHost, st ), "read min/max" ) ) return 1;
@@ -765,7 +781,7 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
     for( int i = 0; i < warmup; i++ )
     {
         if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, c->stream ), "launch" ) ) return 1;
-        if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, c->stream ), "launch" ) ) return 1;
+        if( !HipOk( LaunchProgram( a, dp, c->stream, nullptr ), "launch" ) ) return 1;
     }
     if( !HipOk( hipStreamSynchronize( c->stream ), "warmup" ) ) return 1;
     // The range keys are reset once: every timed launch writes the same
@@ -777,7 +793,7 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
     if( !HipOk( hipEventRecord( c->ev0, c->stream ), "event" ) ) return 1;
     for( int i = 0; i < iters; i++ )
     {
-        if( !HipOk( (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, c->stream ), "launch" ) ) return 1;
+        if( !HipOk( LaunchProgram( a, dp, c->stream, nullptr ), "launch" ) ) return 1;
     }
     if( !HipOk( hipEventRecord( c->ev1, c->stream ), "event" ) ) return 1;
     if( !HipOk( hipStreamSynchronize( c->stream ), "timed launches" ) ) return 1;

@@ -61,6 +61,10 @@ namespace fn2
     const DeviceProgram* GetDeviceProgram( const Program& p );
     bool CompileForDevice( const FastNoise::Generator* root, int dims, const DeviceProgram*& dp, Program* hostCopy );
 
+    // One generator launch: the tree's specialised kernel when it is loaded,
+    // the interpreter otherwise (`specialised` reports which).
+    hipError_t LaunchProgram( const GenArgs& a, const DeviceProgram* dp, hipStream_t stream, bool* specialised );
+
     CallContext* AcquireContext();
     void ReleaseContext( CallContext* c );
     bool IsDevicePointer( const void* p );

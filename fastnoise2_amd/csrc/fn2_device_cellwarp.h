@@ -118,7 +118,7 @@ namespace fn2d
     template<int D>
     FN2D float cellularValue( const CellParams& cp, const float* pos, int valueIndex )
     {
-        float dist[4] = { INFINITY, INFINITY, INFINITY, INFINITY };
+        float dist[4] = { __builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff() };
         int32_t vh[4] = { 0, 0, 0, 0 };
         cellWalk<D, true>( cp, pos, [&]( int32_t h, const float* v, float invMag, const float* fc ) {
             float d[D];
@@ -150,7 +150,7 @@ namespace fn2d
     template<int D>
     FN2D void cellularDistances( const CellParams& cp, const float* pos, float* dist )
     {
-        dist[0] = dist[1] = dist[2] = dist[3] = INFINITY;
+        dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
         cellWalk<D, false>( cp, pos, [&]( int32_t h, const float* v, float invMag, const float* fc ) {
             float d[D];
 #pragma unroll

@@ -8,8 +8,11 @@
 // FS::FMulAdd / FNMulAdd / FMulSub, division and sqrt are the correctly
 // rounded IEEE operations, integer arithmetic wraps (done in uint32_t).
 #pragma once
+// hipRTC (run-time compiled kernels) provides the runtime and the fixed-width types itself
+#if !defined( __HIPCC_RTC__ )
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 // Host-and-device so the same source also compiles as plain host code.
 #define FN2D __host__ __device__ __forceinline__

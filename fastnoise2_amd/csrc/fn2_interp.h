@@ -4,8 +4,11 @@
 // 32-entry register file (uniform indices -> GPR indexing, no scratch).
 // Host-and-device code: the kernels in fn2_kernels.hip instantiate it.
 #pragma once
+// hipRTC (run-time compiled kernels) provides the runtime and the fixed-width types itself
+#if !defined( __HIPCC_RTC__ )
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 #include "fn2_program.h"
 #include "fn2_device_cellwarp.h"
 #include "fn2_kernel_args.h"
@@ -190,14 +193,12 @@ namespace fn2i
     // compiled in (kFeatCell, kFeatWarp, kFeatMisc). The host picks the
     // smallest variant that covers the program, which keeps the VGPR budget
     // (and with it occupancy) of lattice-only programs low.
+    // One instruction. `K` points at the instruction's constants, `seed` is
+    // the call seed plus the instruction's seed delta.
     template<int DM, int FT>
-    FN2D float runProgram( const fn2::GenArgs& a, float* R )
+    FN2D void execIns( const Ins& in, const float* K, const int32_t seed, float* R )
     {
-        for( int pc = 0; pc < a.nIns; pc++ )
         {
-            const Ins in = a.ins[pc];
-            const float* K = a.konst + in.k;
-            const int32_t seed = iadd( a.seed, in.seed );
             const int d = in.dim;
             switch( in.op )
             {
@@ -465,6 +466,16 @@ namespace fn2i
             default:
                 break;
             }
+        }
+    }
+
+    template<int DM, int FT>
+    FN2D float runProgram( const fn2::GenArgs& a, float* R )
+    {
+        for( int pc = 0; pc < a.nIns; pc++ )
+        {
+            const Ins in = a.ins[pc];
+            execIns<DM, FT>( in, a.konst + in.k, iadd( a.seed, in.seed ), R );
         }
         return R[a.resultReg];
     }

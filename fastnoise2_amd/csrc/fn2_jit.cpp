@@ -1,0 +1,337 @@
+// Run-time specialised generator kernels (hipRTC).
+//
+// The interpreter kernels run any compiled tree, fetching each instruction
+// and indexing a 32-entry register file. For a tree that is used repeatedly,
+// this module writes HIP source in which the tree's instructions are
+// compile-time constants: every `execIns` call folds to the one operation it
+// names, register indices become constants, and the register file becomes
+// ordinary VGPRs. The source includes the same device headers as the
+// interpreter (embedded in the library at build time), so the arithmetic is
+// identical and the results are bit-identical.
+//
+// Compilation runs on a background thread (mode async, the default): calls
+// keep using the interpreter until the tree's kernel is loaded. Mode sync
+// compiles on first use; mode off disables specialisation. The mode comes
+// from fn2SetJitMode or the FN2_JIT environment variable (off / async / sync).
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "fn2_backend.h"
+#include "fn2_jit.h"
+
+namespace fn2
+{
+    namespace
+    {
+        struct JitHeader
+        {
+            const char* name;
+            const char* text;
+        };
+
+        const JitHeader kJitHeaders[] = {
+#include "fn2_jit_sources.inc"
+        };
+
+        const char* kJitOptions[] = {
+            "--offload-arch=gfx950", "-O3", "-std=c++17",
+            "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
+            "-fhip-fp32-correctly-rounded-divide-sqrt",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        };
+
+        enum State { kQueued, kReady, kFailed };
+
+        struct Entry
+        {
+            State state = kQueued;
+            JitKernel kernel;
+            std::string error;
+        };
+
+        std::string Source( const DeviceProgram& p )
+        {
+            std::ostringstream s;
+            s << "#include \"fn2_kernel_body.h\"\n"
+                 "using namespace fn2d;\nusing namespace fn2i;\n"
+                 "namespace {\nstruct Prog {\n"
+                 "__device__ __forceinline__ float operator()( const fn2::GenArgs& a, float* R ) const {\n";
+            for( const Ins& in : p.ins )
+            {
+                s << "{ constexpr fn2::Ins in = { " << (int)in.op << ", " << (int)in.dim << ", { ";
+                for( int i = 0; i < 4; i++ ) s << (int)in.p[i] << ( i < 3 ? ", " : " }, { " );
+                for( int i = 0; i < 6; i++ ) s << (int)in.r[i] << ( i < 5 ? ", " : " }, " );
+                s << "(int32_t)" << (uint32_t)in.seed << "u, " << in.k << "u };\n"
+                  << "  execIns<28, 7>( in, a.konst + " << in.k << "u, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
+            }
+            s << "return R[" << p.resultReg << "];\n} };\n}\n"
+                 "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_single( fn2::GenArgs a )\n"
+                 "{ fn2k::generateSingle<256>( a, Prog() ); }\n"
+                 "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_batch( const fn2::GenArgs* __restrict__ jobs, "
+                 "const int32_t* __restrict__ bj, const uint32_t* __restrict__ bb )\n"
+                 "{ fn2k::generateBatch<256>( jobs, bj, bb, Prog() ); }\n";
+            return s.str();
+        }
+
+        bool CompileCode( const DeviceProgram& p, std::vector<char>& code, std::string& err )
+        {
+            std::string src = Source( p );
+            std::vector<const char*> hdrs, names;
+            for( const JitHeader& h : kJitHeaders )
+            {
+                hdrs.push_back( h.text );
+                names.push_back( h.name );
+            }
+            hiprtcProgram prog;
+            if( hiprtcCreateProgram( &prog, src.c_str(), "fn2_jit.hip", (int)hdrs.size(), hdrs.data(), names.data() ) != HIPRTC_SUCCESS )
+            {
+                err = "hiprtcCreateProgram failed";
+                return false;
+            }
+            hiprtcResult r = hiprtcCompileProgram( prog, (int)( sizeof( kJitOptions ) / sizeof( kJitOptions[0] ) ), kJitOptions );
+            if( r != HIPRTC_SUCCESS )
+            {
+                size_t n = 0;
+                hiprtcGetProgramLogSize( prog, &n );
+                std::string log( n, '\0' );
+                if( n ) hiprtcGetProgramLog( prog, &log[0] );
+                err = "hipRTC compile failed: " + log.substr( 0, 2000 );
+                hiprtcDestroyProgram( &prog );
+                return false;
+            }
+            size_t size = 0;
+            hiprtcGetCodeSize( prog, &size );
+            code.resize( size );
+            hiprtcGetCode( prog, code.data() );
+            hiprtcDestroyProgram( &prog );
+            return true;
+        }
+
+        bool Compile( const DeviceProgram& p, JitKernel& k, std::string& err )
+        {
+            std::vector<char> code;
+            if( !CompileCode( p, code, err ) ) return false;
+            hipModule_t mod = nullptr;
+            hipError_t e = hipModuleLoadData( &mod, code.data() );
+            if( e == hipSuccess ) e = hipModuleGetFunction( &k.single, mod, "fn2_jit_single" );
+            if( e == hipSuccess ) e = hipModuleGetFunction( &k.batch, mod, "fn2_jit_batch" );
+            if( e != hipSuccess )
+            {
+                (void)hipGetLastError();
+                err = std::string( "loading the specialised kernel: " ) + hipGetErrorString( e );
+                return false;
+            }
+            k.module = mod;
+            return true;
+        }
+
+        class JitManager
+        {
+        public:
+            JitManager()
+            {
+                const char* env = getenv( "FN2_JIT" );
+                if( env )
+                {
+                    if( !strcmp( env, "off" ) || !strcmp( env, "0" ) ) mode = FN2_JIT_OFF;
+                    else if( !strcmp( env, "sync" ) ) mode = FN2_JIT_SYNC;
+                }
+            }
+
+            ~JitManager() { Shutdown(); }
+
+            // Stops the background compiler (the compilation in progress
+            // finishes first); registered with atexit so that the thread never
+            // outlives the compiler libraries it calls.
+            void Shutdown()
+            {
+                {
+                    std::lock_guard<std::mutex> lk( mu );
+                    stop = true;
+                }
+                cv.notify_all();
+                if( worker.joinable() ) worker.join();
+            }
+
+            int mode = FN2_JIT_ASYNC;
+
+            const JitKernel* Get( const DeviceProgram* p )
+            {
+                if( mode == FN2_JIT_OFF || !p ) return nullptr;
+                std::unique_lock<std::mutex> lk( mu );
+                auto it = entries.find( p );
+                if( it == entries.end() )
+                {
+                    it = entries.emplace( p, Entry() ).first;
+                    if( mode == FN2_JIT_SYNC )
+                    {
+                        lk.unlock();
+                        Entry e;
+                        e.state = Compile( *p, e.kernel, e.error ) ? kReady : kFailed;
+                        lk.lock();
+                        it = entries.find( p );
+                        it->second = e;
+                        cv.notify_all();
+                    }
+                    else
+                    {
+                        if( stop ) return nullptr;
+                        queue.push_back( p );
+                        if( !worker.joinable() ) worker = std::thread( [this] { Work(); } );
+                        cv.notify_all();
+                        return nullptr;
+                    }
+                }
+                else if( mode == FN2_JIT_SYNC && it->second.state == kQueued )
+                {
+                    cv.wait( lk, [&] { return entries[p].state != kQueued; } );
+                    it = entries.find( p );
+                }
+                return it->second.state == kReady ? &it->second.kernel : nullptr;
+            }
+
+            int State( const DeviceProgram* p, std::string* err )
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                auto it = entries.find( p );
+                if( it == entries.end() ) return -1;
+                if( err ) *err = it->second.error;
+                return it->second.state;
+            }
+
+            void Wait()
+            {
+                std::unique_lock<std::mutex> lk( mu );
+                cv.wait( lk, [&] { return queue.empty() && !busy; } );
+            }
+
+        private:
+            void Work()
+            {
+                for( ;; )
+                {
+                    const DeviceProgram* p;
+                    {
+                        std::unique_lock<std::mutex> lk( mu );
+                        cv.wait( lk, [&] { return stop || !queue.empty(); } );
+                        if( stop ) return;
+                        p = queue.front();
+                        queue.pop_front();
+                        busy = true;
+                    }
+                    Entry e;
+                    e.state = Compile( *p, e.kernel, e.error ) ? kReady : kFailed;
+                    {
+                        std::lock_guard<std::mutex> lk( mu );
+                        entries[p] = e;
+                        busy = false;
+                    }
+                    cv.notify_all();
+                }
+            }
+
+            std::mutex mu;
+            std::condition_variable cv;
+            std::unordered_map<const DeviceProgram*, Entry> entries;
+            std::deque<const DeviceProgram*> queue;
+            std::thread worker;
+            bool busy = false;
+            bool stop = false;
+        };
+
+        JitManager* gManager = nullptr;
+
+        void ShutdownAtExit()
+        {
+            if( gManager ) gManager->Shutdown();
+        }
+
+        JitManager& Manager()
+        {
+            // lives for the process (its kernels stay loaded); the worker
+            // thread is stopped at exit
+            static JitManager* m = [] {
+                gManager = new JitManager;
+                std::atexit( ShutdownAtExit );
+                return gManager;
+            }();
+            return *m;
+        }
+    }
+
+    const JitKernel* GetJitKernel( const DeviceProgram* p )
+    {
+        return Manager().Get( p );
+    }
+
+    std::string JitSourceFor( const DeviceProgram& p )
+    {
+        return Source( p );
+    }
+}
+
+extern "C" {
+
+FASTNOISE_API int fn2SetJitMode( int mode )
+{
+    if( mode < FN2_JIT_OFF || mode > FN2_JIT_SYNC ) return -1;
+    int old = fn2::Manager().mode;
+    fn2::Manager().mode = mode;
+    return old;
+}
+
+FASTNOISE_API int fn2JitWait( void )
+{
+    fn2::Manager().Wait();
+    return 0;
+}
+
+FASTNOISE_API int fn2JitCompileCheck( const void* node, int dims, int* codeBytes )
+{
+    fn2::ClearError();
+    if( !node ) { fn2::SetError( "null node" ); return 1; }
+    fn2::Program prog;
+    std::string err;
+    if( !fn2::CompileTree( static_cast<const FastNoise::Generator*>( node ), dims, prog, err ) )
+    {
+        fn2::SetError( "cannot compile node tree: " + err );
+        return 1;
+    }
+    fn2::DeviceProgram dp;
+    dp.ins = prog.ins;
+    dp.konst = prog.konst;
+    dp.resultReg = prog.resultReg;
+    std::vector<char> code;
+    if( !fn2::CompileCode( dp, code, err ) )
+    {
+        fn2::SetError( err );
+        return 1;
+    }
+    if( codeBytes ) *codeBytes = (int)code.size();
+    return 0;
+}
+
+FASTNOISE_API int fn2GetJitState( const void* node, int dims )
+{
+    fn2::ClearError();
+    if( !node ) return -2;
+    const fn2::DeviceProgram* dp = nullptr;
+    if( !fn2::EnsureDevice() || !fn2::CompileForDevice( static_cast<const FastNoise::Generator*>( node ), dims, dp, nullptr ) ) return -2;
+    std::string err;
+    int st = fn2::Manager().State( dp, &err );
+    if( st == 2 ) fn2::SetError( err );
+    return st;
+}
+
+}

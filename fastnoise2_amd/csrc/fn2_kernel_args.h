@@ -1,6 +1,8 @@
 // Kernel argument block of the generator kernels (host and device).
 #pragma once
+#if !defined( __HIPCC_RTC__ )
 #include <stdint.h>
+#endif
 #include "fn2_program.h"
 
 namespace fn2

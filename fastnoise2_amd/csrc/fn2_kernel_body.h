@@ -1,0 +1,112 @@
+// Bodies of the generator kernels, shared by the interpreter kernels
+// (fn2_kernels.hip) and the run-time specialised kernels (fn2_jit.cpp).
+// `prog( args, R )` evaluates one point whose starting position is in R and
+// returns its value: either the interpreter loop or a fixed, inlined program.
+//
+// Output range: every lane keeps a running min/max in registers; a block
+// reduces them through wave shuffles and LDS and publishes one atomic pair.
+// Same-address atomics serialise in L2, so the number of blocks, not of
+// waves, bounds their cost: the single-call kernel runs a grid-stride loop
+// over a grid sized to fill the GPU, and the batched kernel gives each block
+// a tile of several points per lane.
+#pragma once
+#include "fn2_interp.h"
+
+namespace fn2k
+{
+    using namespace fn2d;
+    using namespace fn2i;
+
+    template<int BLOCK>
+    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* keys )
+    {
+        __shared__ float sMin[BLOCK / 64], sMax[BLOCK / 64];
+#pragma unroll
+        for( int off = 32; off > 0; off >>= 1 )
+        {
+            mn = fminf( mn, __shfl_xor( mn, off ) );
+            mx = fmaxf( mx, __shfl_xor( mx, off ) );
+        }
+        const int wave = threadIdx.x >> 6;
+        if( ( threadIdx.x & 63 ) == 0 )
+        {
+            sMin[wave] = mn;
+            sMax[wave] = mx;
+        }
+        __syncthreads();
+        if( threadIdx.x == 0 )
+        {
+#pragma unroll
+            for( int w = 1; w < BLOCK / 64; w++ )
+            {
+                mn = fminf( mn, sMin[w] );
+                mx = fmaxf( mx, sMax[w] );
+            }
+            if( mn <= mx )
+            {
+                atomicMin( &keys[0], orderedKey( mn ) );
+                atomicMax( &keys[1], orderedKey( mx ) );
+            }
+        }
+    }
+
+    // One request: grid-stride loop over all of its points.
+    template<int BLOCK, class Prog>
+    __device__ __forceinline__ void generateSingle( const fn2::GenArgs& a, Prog prog )
+    {
+        float R[fn2::kMaxRegs];
+#pragma unroll
+        for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
+        float mn = __builtin_inff(), mx = -__builtin_inff();
+        const uint32_t total = (uint32_t)a.total;
+        const uint32_t stride = gridDim.x * BLOCK;
+        for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
+        {
+            startPosition( a, i, R );
+            const float v = prog( a, R );
+            a.out[i] = v;
+            if( v == v )
+            {
+                mn = fminf( mn, v );
+                mx = fmaxf( mx, v );
+            }
+        }
+        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+    }
+
+    // Batched requests: block b covers points [blockBase[b], blockBase[b] +
+    // BLOCK * ppt) of request blockJob[b] (clipped to the request's size),
+    // where ppt is the job's points per lane.
+    template<int BLOCK, class Prog>
+    __device__ __forceinline__ void generateBatch( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ blockJob,
+                                                   const uint32_t* __restrict__ blockBase, Prog prog )
+    {
+        // the job is read through a pointer (uniform address -> scalar loads)
+        // rather than copied into a private struct
+        const fn2::GenArgs& a = jobs[blockJob[blockIdx.x]];
+        float R[fn2::kMaxRegs];
+#pragma unroll
+        for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
+        float mn = __builtin_inff(), mx = -__builtin_inff();
+        const uint32_t total = (uint32_t)a.total;
+        const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
+        const int ppt = a.ppt;
+#pragma unroll 1
+        for( int p = 0; p < ppt; p++ )
+        {
+            const uint32_t i = base + p * BLOCK;
+            if( i < total )
+            {
+                startPosition( a, i, R );
+                const float v = prog( a, R );
+                a.out[i] = v;
+                if( v == v )
+                {
+                    mn = fminf( mn, v );
+                    mx = fmaxf( mx, v );
+                }
+            }
+        }
+        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+    }
+}

@@ -15,112 +15,28 @@
 // kernel gives each block a tile of several points per lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "fn2_interp.h"
+#include "fn2_kernel_body.h"
 #include "fn2_kernel_args.h"
 
 using namespace fn2d;
 using namespace fn2i;
 
-namespace
-{
-    template<int BLOCK>
-    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* keys )
-    {
-        __shared__ float sMin[BLOCK / 64], sMax[BLOCK / 64];
-#pragma unroll
-        for( int off = 32; off > 0; off >>= 1 )
-        {
-            mn = fminf( mn, __shfl_xor( mn, off ) );
-            mx = fmaxf( mx, __shfl_xor( mx, off ) );
-        }
-        const int wave = threadIdx.x >> 6;
-        if( ( threadIdx.x & 63 ) == 0 )
-        {
-            sMin[wave] = mn;
-            sMax[wave] = mx;
-        }
-        __syncthreads();
-        if( threadIdx.x == 0 )
-        {
-#pragma unroll
-            for( int w = 1; w < BLOCK / 64; w++ )
-            {
-                mn = fminf( mn, sMin[w] );
-                mx = fmaxf( mx, sMax[w] );
-            }
-            if( mn <= mx )
-            {
-                atomicMin( &keys[0], orderedKey( mn ) );
-                atomicMax( &keys[1], orderedKey( mx ) );
-            }
-        }
-    }
-}
-
-// Single call: grid-stride loop over all points of one request.
+// Single call of the interpreter: grid-stride loop over all points of one request.
 template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate( fn2::GenArgs a )
 {
-    float R[fn2::kMaxRegs];
-#pragma unroll
-    for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
-    float mn = INFINITY, mx = -INFINITY;
-    const uint32_t total = (uint32_t)a.total;
-    const uint32_t stride = gridDim.x * BLOCK;
-    for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
-    {
-        startPosition( a, i, R );
-        const float v = runProgram<DM, FT>( a, R );
-        a.out[i] = v;
-        if( v == v )
-        {
-            mn = fminf( mn, v );
-            mx = fmaxf( mx, v );
-        }
-    }
-    if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+    fn2k::generateSingle<BLOCK>( a, []( const fn2::GenArgs& g, float* R ) { return runProgram<DM, FT>( g, R ); } );
 }
 
 // Batched form for the serving layer: every block looks up its request in a
 // per-block table, so requests with different trees, sizes and sources run in
-// one launch. `jobs` holds one GenArgs per request; block b covers points
-// [blockBase[b], blockBase[b] + BLOCK * ppt) of request blockJob[b] (clipped
-// to the request's size), where ppt is the job's points per lane (chosen per
-// tree by the serving engine's tuner).
+// one launch (points per lane chosen per tree by the serving engine's tuner).
 template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ blockJob, const uint32_t* __restrict__ blockBase )
 {
-    // read the job through a pointer (uniform address -> scalar loads) rather
-    // than copying it into a private struct
-    const fn2::GenArgs& a = jobs[blockJob[blockIdx.x]];
-    float R[fn2::kMaxRegs];
-#pragma unroll
-    for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
-    float mn = INFINITY, mx = -INFINITY;
-    const uint32_t total = (uint32_t)a.total;
-    const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
-    const int ppt = a.ppt;
-#pragma unroll 1
-    for( int p = 0; p < ppt; p++ )
-    {
-        const uint32_t i = base + p * BLOCK;
-        if( i < total )
-        {
-            startPosition( a, i, R );
-            const float v = runProgram<DM, FT>( a, R );
-            a.out[i] = v;
-            if( v == v )
-            {
-                mn = fminf( mn, v );
-                mx = fmaxf( mx, v );
-            }
-        }
-    }
-    if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+    fn2k::generateBatch<BLOCK>( jobs, blockJob, blockBase, []( const fn2::GenArgs& g, float* R ) { return runProgram<DM, FT>( g, R ); } );
 }
 
-// Applies one of the device math routines to an array (op 0: sqrtPosRN,
-// 1: recipPosRN, 2: invSqrtPos, 3: the IEEE 1 / sqrt).
 __global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,6 +91,13 @@ extern "C" {
     X( 4, 0 ) X( 8, 0 ) X( 16, 0 ) \
     X( 4, 7 ) X( 8, 7 ) X( 16, 7 ) \
     X( 28, 7 )
+
+// blocks of a single-call launch over `total` points (also used for the
+// run-time specialised kernels)
+unsigned fn2_grid_for( int64_t total )
+{
+    return GridFor( total );
+}
 
 // threads per block of the batched kernel: a job with ppt points per lane
 // covers kBlock * ppt points per block

@@ -9,7 +9,19 @@
 // instruction (`seed` field). Node parameters live in a float constant pool
 // (`k` indexes it); small integer parameters ride in `p[]`.
 #pragma once
+#if !defined( __HIPCC_RTC__ )
 #include <stdint.h>
+#else
+// hipRTC has no <stdint.h>
+typedef signed char int8_t;
+typedef unsigned char uint8_t;
+typedef short int16_t;
+typedef unsigned short uint16_t;
+typedef int int32_t;
+typedef unsigned int uint32_t;
+typedef long long int64_t;
+typedef unsigned long long uint64_t;
+#endif
 
 namespace fn2
 {

@@ -40,6 +40,7 @@
 
 #include "fn2_backend.h"
 #include "fn2_internal.h"
+#include "fn2_jit.h"
 
 extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
@@ -83,6 +84,8 @@ namespace
         size_t outOffset = 0;
         int job = 0;
         int ppt = 1;
+        const fn2::JitKernel* jit = nullptr;
+        bool direct = false;   // hostOut is pinned: copied device -> caller directly
         // results
         bool done = false;
         int status = 0;
@@ -108,7 +111,7 @@ namespace
         // graph mode
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        std::vector<std::pair<int, int>> graphShape;   // (variant key, blocks) per launch
+        std::vector<std::pair<int64_t, int>> graphShape;   // (kernel key, blocks) per launch
         int graphJobs = -1;
     };
 
@@ -251,6 +254,12 @@ namespace
                 s.dBlockBase = (uint32_t*)( (uint8_t*)s.dJobs + jobsBytes + bjBytes );
             }
             deviceBytes = (int64_t)( perSlot * nSlots );
+            {
+                int dev = 0, cus = 0;
+                if( hipGetDevice( &dev ) == hipSuccess && hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) == hipSuccess && cus > 0 )
+                    fillBlocks = 2 * cus;
+                (void)hipGetLastError();
+            }
             if( mode != FN2_SERVE_PLAIN )
             {
                 size_t tb = (size_t)kTunePoints / kThreads + 1;
@@ -345,6 +354,14 @@ namespace
             r->tree = treeId;
             r->spec = spec;
             r->hostOut = hostOut;
+            // pinned destinations receive their values straight from the device
+            // (not in graph mode, whose recorded copies must not depend on the caller)
+            if( mode != FN2_SERVE_GRAPH )
+            {
+                hipPointerAttribute_t attr;
+                if( hipPointerGetAttributes( &attr, hostOut ) == hipSuccess ) r->direct = attr.type == hipMemoryTypeHost;
+                else (void)hipGetLastError();
+            }
             std::string err;
             if( !fn2::FillSpecArgs( spec, r->args, r->points, err ) )
             {
@@ -538,6 +555,7 @@ namespace
         int64_t completed = 0;
         int64_t deviceBytes = 0;
         int maxBlocks = 0;
+        int fillBlocks = 512;   // blocks that fill the GPU twice over (2 per CU)
         int nextSlot = 0;
         std::thread dispatcher, completer;
         std::unique_ptr<CopyPool> copies;
@@ -562,19 +580,32 @@ namespace
             }
         }
 
-        static int VariantKey( const fn2::DeviceProgram* p ) { return p->dimMask * 16 + p->featMask; }
+        // Kernel of a job: the tree's specialised batch kernel when it is loaded
+        // (key = its address), else the interpreter variant (key < 256).
+        static int64_t KernelKey( const fn2::DeviceProgram* p, const fn2::JitKernel* k )
+        {
+            return k ? (int64_t)(uintptr_t)k : p->dimMask * 16 + p->featMask;
+        }
 
         bool Launch( Slot& s )
         {
-            // jobs sorted by kernel variant so that each variant's blocks are contiguous
+            // jobs sorted by kernel so that each kernel's blocks are contiguous
             std::vector<Request*> order( s.reqs );
-            std::stable_sort( order.begin(), order.end(), []( Request* a, Request* b ) { return VariantKey( a->prog ) < VariantKey( b->prog ); } );
+            for( Request* r : order ) r->jit = fn2::GetJitKernel( r->prog );
+            std::stable_sort( order.begin(), order.end(),
+                              []( Request* a, Request* b ) { return KernelKey( a->prog, a->jit ) < KernelKey( b->prog, b->jit ); } );
             GenArgs* hJobs = (GenArgs*)s.hTables;
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             int32_t* hBJ = (int32_t*)( s.hTables + jobsBytes );
             uint32_t* hBB = (uint32_t*)( s.hTables + jobsBytes + bjBytes );
-            std::vector<std::pair<int, int>> shape;   // (variant key, blocks)
+            std::vector<std::pair<int64_t, int>> shape;   // (kernel key, blocks)
+            std::vector<const fn2::JitKernel*> kernels;   // per entry of shape
+            // A batch too small to fill the GPU at the tuned points per lane
+            // runs with fewer points per lane (more blocks): the tuned value
+            // maximises throughput, not the latency of a lone small request.
+            int pptCap = 1;
+            while( pptCap < 8 && (int64_t)pptCap * 2 * kThreads * fillBlocks <= s.points ) pptCap *= 2;
             int nb = 0;
             for( size_t j = 0; j < order.size(); j++ )
             {
@@ -582,18 +613,23 @@ namespace
                 r->job = (int)j;
                 GenArgs a = r->args;
                 a.out = s.dOut + r->outOffset;
-                a.ppt = r->ppt;
+                const int ppt = std::min( r->ppt, pptCap );
+                a.ppt = ppt;
                 a.minmax = s.dKeys + 2 * j;
                 hJobs[j] = a;
-                const int64_t tile = (int64_t)kThreads * r->ppt;
+                const int64_t tile = (int64_t)kThreads * ppt;
                 int rb = (int)( ( r->points + tile - 1 ) / tile );
                 for( int b = 0; b < rb; b++ )
                 {
                     hBJ[nb + b] = (int32_t)j;
                     hBB[nb + b] = (uint32_t)( b * tile );
                 }
-                int key = VariantKey( r->prog );
-                if( shape.empty() || shape.back().first != key ) shape.emplace_back( key, rb );
+                int64_t key = KernelKey( r->prog, r->jit );
+                if( shape.empty() || shape.back().first != key )
+                {
+                    shape.emplace_back( key, rb );
+                    kernels.push_back( r->jit );
+                }
                 else shape.back().second += rb;
                 nb += rb;
             }
@@ -611,7 +647,7 @@ namespace
                 if( !s.exec )
                 {
                     if( !fn2::HipOk( hipStreamBeginCapture( s.stream, hipStreamCaptureModeThreadLocal ), "begin capture" ) ) return false;
-                    bool ok = Enqueue( s, shape, nJobs, nb );
+                    bool ok = Enqueue( s, shape, kernels, nJobs, nb );
                     hipGraph_t g = nullptr;
                     hipError_t e = hipStreamEndCapture( s.stream, &g );
                     if( !ok || !fn2::HipOk( e, "end capture" ) ) return false;
@@ -622,7 +658,7 @@ namespace
                 }
                 if( !fn2::HipOk( hipGraphLaunch( s.exec, s.stream ), "graph launch" ) ) return false;
             }
-            else if( !Enqueue( s, shape, nJobs, nb ) ) return false;
+            else if( !Enqueue( s, shape, kernels, nJobs, nb ) ) return false;
             if( !fn2::HipOk( hipEventRecord( s.done, s.stream ), "serve event" ) ) return false;
             s.reqs.swap( order );
             return true;
@@ -630,7 +666,8 @@ namespace
 
         // Stream work of one batch: table upload, range-key reset, one batch
         // launch per kernel variant, copies of outputs and keys to pinned memory.
-        bool Enqueue( Slot& s, const std::vector<std::pair<int, int>>& shape, int nJobs, int nb )
+        bool Enqueue( Slot& s, const std::vector<std::pair<int64_t, int>>& shape, const std::vector<const fn2::JitKernel*>& kernels,
+                      int nJobs, int nb )
         {
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
@@ -640,15 +677,39 @@ namespace
             if( !fn2::HipOk( hipMemcpyAsync( s.dBlockBase, s.hTables + jobsBytes + bjBytes, sizeof( uint32_t ) * nb, hipMemcpyHostToDevice, s.stream ), "upload blocks" ) ) return false;
             if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, s.stream ), "range reset" ) ) return false;
             int first = 0;
-            for( auto& g : shape )
+            for( size_t gi = 0; gi < shape.size(); gi++ )
             {
-                int dm = g.first / 16, fm = g.first % 16;
-                if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, s.dBlockJob + first, s.dBlockBase + first, g.second, dm, fm, s.stream ), "batch launch" ) ) return false;
+                const auto& g = shape[gi];
+                const int32_t* bj = s.dBlockJob + first;
+                const uint32_t* bb = s.dBlockBase + first;
+                if( kernels[gi] )
+                {
+                    const fn2::GenArgs* jobs = s.dJobs;
+                    void* params[] = { (void*)&jobs, (void*)&bj, (void*)&bb };
+                    if( !fn2::HipOk( hipModuleLaunchKernel( kernels[gi]->batch, g.second, 1, 1, kThreads, 1, 1, 0, s.stream, params, nullptr ),
+                                     "specialised batch launch" ) )
+                        return false;
+                }
+                else
+                {
+                    int dm = (int)( g.first / 16 ), fm = (int)( g.first % 16 );
+                    if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, bj, bb, g.second, dm, fm, s.stream ), "batch launch" ) ) return false;
+                }
                 first += g.second;
             }
             // the requests occupy [0, points) of the arena
             size_t outBytes = (size_t)s.points * 4;
-            if( !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, outBytes, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
+            bool staged = false;
+            for( Request* r : s.reqs )
+            {
+                if( r->direct )
+                {
+                    if( !fn2::HipOk( hipMemcpyAsync( r->hostOut, s.dOut + r->outOffset, (size_t)r->points * 4, hipMemcpyDeviceToHost, s.stream ), "copy output" ) )
+                        return false;
+                }
+                else staged = true;
+            }
+            if( staged && !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, outBytes, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
             if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, 2 * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
             return true;
         }
@@ -702,7 +763,8 @@ namespace
                 {
                     std::vector<CopyPool::Seg> segs;
                     segs.reserve( s.reqs.size() );
-                    for( Request* r : s.reqs ) segs.push_back( { r->hostOut, s.hOut + r->outOffset, (size_t)r->points } );
+                    for( Request* r : s.reqs )
+                        if( !r->direct ) segs.push_back( { r->hostOut, s.hOut + r->outOffset, (size_t)r->points } );
                     copies->Run( segs );
                     for( Request* r : s.reqs )
                     {
@@ -831,7 +893,17 @@ FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, c
         }
         maxPoints = std::max( maxPoints, n );
     }
-    std::vector<std::vector<float>> pool( (size_t)maxOutstanding, std::vector<float>( (size_t)std::max<int64_t>( maxPoints, 1 ) ) );
+    // pinned buffers, as a serving front end would hold: the engine copies into them directly
+    std::vector<float*> pool( (size_t)maxOutstanding, nullptr );
+    struct PoolFree
+    {
+        std::vector<float*>& p;
+        ~PoolFree() { for( float* b : p ) if( b ) (void)hipHostFree( b ); }
+    } poolFree{ pool };
+    for( float*& b : pool )
+    {
+        if( !fn2::HipOk( hipHostMalloc( (void**)&b, (size_t)std::max<int64_t>( maxPoints, 1 ) * 4, hipHostMallocDefault ), "replay buffers" ) ) return 1;
+    }
     std::deque<std::pair<int64_t, int>> pending;   // (request id, pool slot), in submit order
     std::vector<int> freeSlots;
     for( int k = maxOutstanding - 1; k >= 0; k-- ) freeSlots.push_back( k );
@@ -860,7 +932,7 @@ FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, c
         while( freeSlots.empty() ) retireOne();
         int slot = freeSlots.back();
         freeSlots.pop_back();
-        int64_t id = e->Submit( treeIds[i], specs[i], pool[slot].data() );
+        int64_t id = e->Submit( treeIds[i], specs[i], pool[slot] );
         if( id < 0 ) return 1;
         pending.emplace_back( id, slot );
         reqIndex.push_back( i );

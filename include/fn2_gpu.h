@@ -68,6 +68,30 @@ FASTNOISE_API const char* fn2GetLastError( void );
  * 1/sqrt. Lets a test compare them with IEEE results on the installed GPU. */
 FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n );
 
+/* Run-time specialisation of trees (hipRTC): a tree used repeatedly gets a
+ * kernel in which its instructions are compile-time constants (bit-identical
+ * results, no interpretation). OFF: interpreter only. ASYNC (default):
+ * compile in the background on first use, interpreter meanwhile. SYNC:
+ * compile on first use. The FN2_JIT environment variable (off / async /
+ * sync) sets the initial mode. fn2SetJitMode returns the previous mode. */
+enum fn2JitMode
+{
+    FN2_JIT_OFF = 0,
+    FN2_JIT_ASYNC = 1,
+    FN2_JIT_SYNC = 2,
+};
+FASTNOISE_API int fn2SetJitMode( int mode );
+/* Writes and compiles (hipRTC, no GPU needed) a tree's specialised kernel
+ * without loading it: 0 on success (codeBytes = size of the code object),
+ * nonzero with the compiler log in fn2GetLastError otherwise. */
+FASTNOISE_API int fn2JitCompileCheck( const void* node, int dims, int* codeBytes );
+/* Waits until every queued specialisation has finished compiling. */
+FASTNOISE_API int fn2JitWait( void );
+/* State of a tree's specialised kernel in `dims` dimensions: -1 never
+ * requested, 0 queued, 1 ready, 2 failed (fn2GetLastError holds the
+ * compiler log), -2 bad arguments. */
+FASTNOISE_API int fn2GetJitState( const void* node, int dims );
+
 /* Compiled-program facts for a tree evaluated in `dims` dimensions. */
 FASTNOISE_API int fn2GetProgramInfo( const void* node, int dims, int* instructions, int* registers, int* dimMask, int* featureMask );
 
@@ -118,8 +142,9 @@ FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims );
 /* Replays a request trace against the engine from the calling thread:
  * request i (tree treeIds[i], spec specs[i]) is submitted at arrivalSec[i]
  * seconds after the start (open loop; NULL = as fast as possible), with at
- * most maxOutstanding requests in flight (closed-loop bound; outputs go to an
- * internal buffer pool). latencyMs[i] receives request i's submit-to-done
+ * most maxOutstanding requests in flight (closed-loop bound; outputs go to a
+ * pool of pinned host buffers, which the engine fills straight from the
+ * device). latencyMs[i] receives request i's submit-to-done
  * time, wallSec the time from the start to the last completion. */
 FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,
                                   int maxOutstanding, double* latencyMs, double* wallSec );

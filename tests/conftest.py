@@ -19,6 +19,8 @@ def gpu():
     import fastnoise2_amd as fn
     n = fn.device_count()
     assert n > 0, "no HIP device visible: GPU tests need an MI355X"
+    # the interpreter by default; tests/test_gpu_jit.py switches specialisation on
+    fn.set_jit_mode(fn.JIT_OFF)
     return n
 
 
