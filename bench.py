@@ -31,23 +31,6 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes per clock x 2.4 
 HBM_PEAK_GBS = 8000.0
 
 
-def workloads(W):
-    return {
-        "fbm_simplex_2d_grid_4096x4096": dict(tree=W.fbm_simplex_2d, kind="grid", dims=2, offset=[-2048, -2048],
-                                              count=[4096, 4096], seed=42),
-        "fbm_simplex_tileable_1024x1024": dict(tree=W.fbm_simplex_2d, kind="tileable", dims=2, count=[1024, 1024], seed=42),
-        "perlin_3d_grid_64x64x64": dict(tree=W.perlin_3d, kind="grid", dims=3, offset=[0, 0, 0], count=[64, 64, 64], seed=1337),
-        "warped_cellular_3d_grid_256x256x256": dict(tree=W.warped_cellular_3d, kind="grid", dims=3, offset=[0, 0, 0],
-                                                    count=[256, 256, 256], seed=1337),
-        "warped_cellular_positions_16777216": dict(tree=W.warped_cellular_3d, kind="positions", dims=3, count=[1 << 24],
-                                                   seed=1337),
-        "serving_tree_3d_grid_256x256x256": dict(tree=W.serving_tree, kind="grid", dims=3, offset=[0, 0, 0],
-                                                 count=[256, 256, 256], seed=1337),
-        "mixed_tree_deepest_3d_grid_128x128x128": dict(tree=lambda: W.decode(W.mixed_trees()[-1]), kind="grid", dims=3,
-                                                       offset=[0, 0, 0], count=[128, 128, 128], seed=1337),
-    }
-
-
 def make_spec(fn, W, wl, keep):
     if wl["kind"] == "grid":
         return fn.grid_spec(wl["dims"], wl["offset"], wl["count"], seed=wl["seed"])
@@ -61,40 +44,64 @@ def make_spec(fn, W, wl, keep):
 
 def cpu_baseline(wl, enc, target_s=12.0):
     """Times the oracle (16 lanes, one thread per core of this process's CPU
-    share, at most 16) on a bounded slice of the workload."""
+    share, at most 16) on a bounded sample of the workload: a slice of it when
+    the whole takes longer than target_s, else whole passes repeated for about
+    target_s."""
     import oracle
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     t = oracle.Tree(enc)
 
-    def run(rows):
-        if wl["kind"] == "grid":
-            count = list(wl["count"])
-            count[-1] = rows
-            n = int(np.prod(count))
-            t0 = time.perf_counter()
-            t.grid(wl["offset"], count, seed=wl["seed"], lanes=16, threads=threads)
-            return n, time.perf_counter() - t0, "GenUniformGrid%dD slice %s" % (wl["dims"], "x".join(map(str, count)))
-        if wl["kind"] == "tileable":
-            n = wl["count"][0] * rows
-            t0 = time.perf_counter()
-            t.tileable(wl["count"][0], rows, seed=wl["seed"], lanes=16, threads=threads)
-            return n, time.perf_counter() - t0, "GenTileable2D %dx%d" % (wl["count"][0], rows)
-        import fastnoise2_amd.workloads as W
-        n = rows * 4096
-        x, y, z = W.splitmix64_positions(n, seed=7)
+    def run(rows, reps=1):
         t0 = time.perf_counter()
-        t.positions([x, y, z], seed=wl["seed"], lanes=16, threads=threads)
-        return n, time.perf_counter() - t0, "GenPositionArray3D first %d positions" % n
+        for _ in range(reps):
+            if wl["kind"] == "grid":
+                count = list(wl["count"])
+                count[-1] = rows
+                t.grid(wl["offset"], count, seed=wl["seed"], lanes=16, threads=threads)
+                what = "GenUniformGrid%dD %s" % (wl["dims"], "x".join(map(str, count)))
+            elif wl["kind"] == "tileable":
+                t.tileable(wl["count"][0], rows, seed=wl["seed"], lanes=16, threads=threads)
+                what = "GenTileable2D %dx%d" % (wl["count"][0], rows)
+            else:
+                t.positions(pos[:rows * 4096], seed=wl["seed"], lanes=16, threads=threads)
+                what = "GenPositionArray3D first %d positions" % (rows * 4096)
+        n = reps * rows * (int(np.prod(wl["count"][:-1])) if wl["kind"] != "positions" else 4096)
+        return n, time.perf_counter() - t0, what if reps == 1 else "%s x %d passes" % (what, reps)
 
+    if wl["kind"] == "positions":
+        import fastnoise2_amd.workloads as W
+        pos = np.stack(W.splitmix64_positions(wl["count"][0], seed=7))
+        pos = [pos[0], pos[1], pos[2]]
+        pos = _RowView(pos)
     full_rows = wl["count"][-1] if wl["kind"] != "positions" else wl["count"][0] // 4096
     rows = max(1, full_rows // 64)
-    n, dt, what = run(rows)
-    rate = n / max(dt, 1e-9)
-    per_row = n / rows
-    rows = int(min(full_rows, max(rows, target_s * rate / per_row)))
-    n, dt, what = run(rows)
+    while True:
+        n, dt, what = run(rows)
+        if dt >= 0.5 or rows == full_rows:
+            break
+        rows = min(full_rows, rows * 4)
+    per_row_s = max(dt, 1e-9) / rows
+    if per_row_s * full_rows >= target_s:
+        n, dt, what = run(max(rows, int(target_s / per_row_s)))
+    else:
+        n, dt, what = run(full_rows, max(1, int(round(target_s / (per_row_s * full_rows)))))
     return {"value": n / dt / 1e9, "unit": "Gpoints/s", "cores": threads, "kind": "port",
             "sample": "%s (%d points, %.1f s), CPU oracle 16-lane SIMD, %d threads" % (what, n, dt, threads)}
+
+
+class _RowView:
+    """Slices each of the x, y, z position arrays alike."""
+
+    def __init__(self, arrays):
+        self.arrays = arrays
+
+    def __getitem__(self, sl):
+        return [a[sl] for a in self.arrays]
+
+
+def progress(msg):
+    """One progress line on stderr per finished phase (the JSON line stays alone on stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
@@ -102,6 +109,7 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
     the Poisson trace (open loop) for p50/p99 latency, and a saturating closed
     loop (512 outstanding) of the same request mix for throughput, in the
     plain (one launch + copy per chunk), batched and graph-replay modes."""
+    import fastnoise2_amd as fn
     from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
     enc = W.serving_tree().encode()
     specs, arrivals = W.serving_trace(trace_requests, rate)
@@ -112,6 +120,7 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tid = eng.add_tree(enc)
         eng.replay([tid] * 64, specs[:64], None, 16)   # compile + warm up
+        fn.jit_wait()   # specialised kernels ready before timing
         rec = {}
         if name != "graph":
             lat, wall = eng.replay([tid] * len(specs), specs, arrivals, 512)
@@ -126,6 +135,7 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
                     "batches": eng.stats()["batches"]})
         eng.close()
         out[name] = rec
+        progress("serving %s done" % name)
     out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
     return out
 
@@ -134,6 +144,7 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
     """The mixed-tree serving workload: 8 seeded random trees (depth 3..12),
     2D/3D/4D requests of 2^12..2^22 points, bursty on/off arrivals (10 ms at
     10k req/s); per-tree tuned batched launches vs graph replay vs plain."""
+    import fastnoise2_amd as fn
     from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
     encs = W.mixed_trees()
     trees, specs, arrivals = W.mixed_trace(trace_requests)
@@ -146,6 +157,7 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
         tids = [eng.add_tree(e) for e in encs]
         wt = [t for t in tids for _ in warm]
         eng.replay(wt, warm * len(tids), None, 8)   # compile + tune every tree and dimensionality
+        fn.jit_wait()
         rec = {}
         lat, wall = eng.replay([tids[t] for t in trees], specs, arrivals, 512)
         rec.update({"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -159,6 +171,7 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
             rec["tuned_points_per_lane"] = [[eng.tree_tuning(t, d) for d in (2, 3, 4)] for t in tids]
         eng.close()
         out[name] = rec
+        progress("mixed serving %s done" % name)
     out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
     return out
 
@@ -194,7 +207,7 @@ def main():
                 raise SystemExit("copy failed: " + fn.last_error())
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return
-    wls = workloads(W)
+    wls = W.bench_workloads()
     wl = wls[args.workload]
     spec_tree = wl["tree"]()
     enc = spec_tree.encode()
@@ -217,17 +230,12 @@ def main():
             kernel_name = "fn2_jit_single"
     ms_event, ms_wall = node.bench_device(spec, out, warmup=args.warmup, iters=args.steps)
     value = n / (ms_wall * 1e-3) / 1e9
+    fn.set_jit_mode(fn.JIT_ASYNC)
+    progress("%s: %d timed steps done, %.2f Gpoints/s" % (args.workload, args.steps, value))
 
-    # algorithmic work per point from the op-counting oracle
-    import oracle
-    if wl["kind"] == "grid":
-        sample = [min(c, 32) for c in wl["count"]]
-        flop_pt, valu_pt = oracle.op_counts_per_point(enc, wl["dims"], wl["offset"], sample, seed=wl["seed"])
-    elif wl["kind"] == "tileable":
-        flop_pt, valu_pt = oracle.op_counts_per_point(enc, 2, None, [64, 64], seed=wl["seed"], tileable=True)
-    else:
-        flop_pt, valu_pt = oracle.op_counts_per_point(enc, 3, [-1000, -1000, -1000], [16, 16, 16], step=[125, 125, 125],
-                                                      seed=wl["seed"])
+    # algorithmic work per point (counted by the op-counting oracle, committed
+    # with the workload and recounted by the CPU tests)
+    flop_pt, valu_pt = wl["ops"]
     kernel_s = ms_event * 1e-3
     tflops = flop_pt * n / kernel_s / 1e12
     bytes_pt = 4 + (4 * wl["dims"] if wl["kind"] == "positions" else 0)
@@ -278,6 +286,7 @@ def main():
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(wl, enc, args.cpu_seconds)
         rec["gpu_vs_cpu"] = round(value / rec["cpu_baseline"]["value"], 1)
+        progress("cpu baseline done")
     if not args.no_serving and args.workload == "fbm_simplex_2d_grid_4096x4096":
         rec["serving"] = serving_bench(W)
         rec["serving_mixed"] = serving_mixed_bench(W)

@@ -246,8 +246,17 @@ class Node:
         return ms.value, wall.value
 
     def jit_state(self, dims):
-        """-1 never requested, 0 queued, 1 ready, 2 failed."""
+        """-1 never requested, 0 queued, 1 ready, 2 failed, 3 kept on the interpreter (too long)."""
         return lib().fn2GetJitState(self.handle, dims)
+
+    def jit_source(self, dims):
+        """HIP source of the tree's specialised kernel."""
+        n = lib().fn2JitSource(self.handle, dims, None, 0)
+        if n < 0:
+            raise RuntimeError("fn2JitSource failed: " + last_error())
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().fn2JitSource(self.handle, dims, buf, n + 1)
+        return buf.value.decode()
 
     def program_info(self, dims):
         ins, regs, dm, fm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

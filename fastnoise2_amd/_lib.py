@@ -102,6 +102,7 @@ GPU_EXT = {
     "fn2SetJitMode": (i, [i]),
     "fn2JitWait": (i, []),
     "fn2JitCompileCheck": (i, [vp, i, c_int_p]),
+    "fn2JitSource": (i, [vp, i, ctypes.c_char_p, i]),
     "fn2GetJitState": (i, [vp, i]),
     "fn2GetProgramInfo": (i, [vp, i, c_int_p, c_int_p, c_int_p, c_int_p]),
     "fn2ServeCreate": (vp, [ctypes.c_size_t, i, i]),

@@ -31,65 +31,59 @@ namespace fn2i
         return b >= 0 ? b : b ^ 0x7fffffff;
     }
 
-    template<int D>
-    FN2D void leafAt( const Ins& in, const float* K, int32_t seed, const float* R, int p, float& out )
-    {
-        const float freq = K[0];
-        float v;
-        switch( in.op )
-        {
-        case fn2::OP_PERLIN:
-            if constexpr( D == 2 ) v = perlin2( seed, R[p] * freq, R[p + 1] * freq );
-            else if constexpr( D == 3 ) v = perlin3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
-            else v = perlin4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
-            break;
-        case fn2::OP_VALUE:
-            if constexpr( D == 2 ) v = value2( seed, R[p] * freq, R[p + 1] * freq );
-            else if constexpr( D == 3 ) v = value3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
-            else v = value4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
-            break;
-        case fn2::OP_SIMPLEX:
-            if constexpr( D == 2 ) v = simplex2( seed, R[p] * freq, R[p + 1] * freq );
-            else if constexpr( D == 3 ) v = simplex3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
-            else v = simplex4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
-            break;
-        default:
-            if constexpr( D == 2 ) v = superSimplex2( seed, R[p] * freq, R[p + 1] * freq );
-            else if constexpr( D == 3 ) v = superSimplex3( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq );
-            else v = superSimplex4( seed, R[p] * freq, R[p + 1] * freq, R[p + 2] * freq, R[p + 3] * freq );
-            break;
-        }
-        out = scaleOutput( v, K[1], K[2], K[3] );
-    }
+    // Compile-time op selection. The interpreter passes OP = -1 (any op, chosen
+    // at run time from the instruction); a specialised kernel passes the
+    // instruction's own op, so only that op's code is instantiated.
+    template<int OP, int... Ops>
+    constexpr bool opIn() { return OP < 0 || ( ( OP == Ops ) || ... ); }
 
-    // Lattice noise on explicit coordinates (used by the fused fractal loop)
-    template<int D>
+    // Lattice noise on explicit coordinates; LEAF as OP above
+    template<int D, int LEAF = -1>
     FN2D float leafRaw( int op, int32_t seed, const float* x )
     {
-        switch( op )
+        if constexpr( opIn<LEAF, fn2::OP_PERLIN>() )
+            if( LEAF >= 0 || op == fn2::OP_PERLIN )
+            {
+                if constexpr( D == 2 ) return perlin2( seed, x[0], x[1] );
+                else if constexpr( D == 3 ) return perlin3( seed, x[0], x[1], x[2] );
+                else return perlin4( seed, x[0], x[1], x[2], x[3] );
+            }
+        if constexpr( opIn<LEAF, fn2::OP_VALUE>() )
+            if( LEAF >= 0 || op == fn2::OP_VALUE )
+            {
+                if constexpr( D == 2 ) return value2( seed, x[0], x[1] );
+                else if constexpr( D == 3 ) return value3( seed, x[0], x[1], x[2] );
+                else return value4( seed, x[0], x[1], x[2], x[3] );
+            }
+        if constexpr( opIn<LEAF, fn2::OP_SIMPLEX>() )
+            if( LEAF >= 0 || op == fn2::OP_SIMPLEX )
+            {
+                if constexpr( D == 2 ) return simplex2( seed, x[0], x[1] );
+                else if constexpr( D == 3 ) return simplex3( seed, x[0], x[1], x[2] );
+                else return simplex4( seed, x[0], x[1], x[2], x[3] );
+            }
+        if constexpr( opIn<LEAF, fn2::OP_SUPERSIMPLEX>() )
         {
-        case fn2::OP_PERLIN:
-            if constexpr( D == 2 ) return perlin2( seed, x[0], x[1] );
-            else if constexpr( D == 3 ) return perlin3( seed, x[0], x[1], x[2] );
-            else return perlin4( seed, x[0], x[1], x[2], x[3] );
-        case fn2::OP_VALUE:
-            if constexpr( D == 2 ) return value2( seed, x[0], x[1] );
-            else if constexpr( D == 3 ) return value3( seed, x[0], x[1], x[2] );
-            else return value4( seed, x[0], x[1], x[2], x[3] );
-        case fn2::OP_SIMPLEX:
-            if constexpr( D == 2 ) return simplex2( seed, x[0], x[1] );
-            else if constexpr( D == 3 ) return simplex3( seed, x[0], x[1], x[2] );
-            else return simplex4( seed, x[0], x[1], x[2], x[3] );
-        default:
             if constexpr( D == 2 ) return superSimplex2( seed, x[0], x[1] );
             else if constexpr( D == 3 ) return superSimplex3( seed, x[0], x[1], x[2] );
             else return superSimplex4( seed, x[0], x[1], x[2], x[3] );
         }
+        return 0.0f;
+    }
+
+    template<int D, int OP = -1>
+    FN2D void leafAt( const Ins& in, const float* K, int32_t seed, const float* R, int p, float& out )
+    {
+        const float freq = K[0];
+        float x[D];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) x[a] = R[p + a] * freq;
+        out = scaleOutput( leafRaw<D, OP>( in.op, seed, x ), K[1], K[2], K[3] );
     }
 
     // FractalFBm / FractalRidged over a lattice leaf with constant gain and
     // weighted strength (Fractal.inl:9-67); the octave loop stays in registers.
-    template<int D>
+    template<int D, int LEAF = -1>
     FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, const float* R, int p )
     {
         const int leaf = in.p[0], octaves = in.p[1];
@@ -103,7 +97,7 @@ namespace fn2i
             float x[D];
 #pragma unroll
             for( int a = 0; a < D; a++ ) x[a] = pos[a] * freq;
-            return scaleOutput( leafRaw<D>( leaf, s, x ), nMin, oScale, rMin );
+            return scaleOutput( leafRaw<D, LEAF>( leaf, s, x ), nMin, oScale, rMin );
         };
         float amp = 1.0f;
         float noise = octave( seed );
@@ -127,7 +121,7 @@ namespace fn2i
         return sum;
     }
 
-    template<int D>
+    template<int D, int OP = -1>
     FN2D void cellular( const Ins& in, const float* K, int32_t seed, float* R )
     {
         CellParams cp;
@@ -142,17 +136,21 @@ namespace fn2i
         const int p = in.r[1];
 #pragma unroll
         for( int a = 0; a < D; a++ ) pos[a] = R[p + a] * freq;
-        if( in.op == fn2::OP_CELL_VALUE )
-        {
-            R[in.r[0]] = scaleOutput( cellularValue<D>( cp, pos, in.p[1] ), K[1], K[2], K[4] );
-        }
-        else if( in.op == fn2::OP_CELL_DISTANCE )
-        {
-            float dist[4];
-            cellularDistances<D>( cp, pos, dist );
-            R[in.r[0]] = scaleOutput( cellularReturn( dist, in.p[0], in.p[1], in.p[2], in.p[3], 0.0f ), K[1], K[2], K[4] );
-        }
-        else
+        if constexpr( opIn<OP, fn2::OP_CELL_VALUE>() )
+            if( in.op == fn2::OP_CELL_VALUE )
+            {
+                R[in.r[0]] = scaleOutput( cellularValue<D>( cp, pos, in.p[1] ), K[1], K[2], K[4] );
+                return;
+            }
+        if constexpr( opIn<OP, fn2::OP_CELL_DISTANCE>() )
+            if( in.op == fn2::OP_CELL_DISTANCE )
+            {
+                float dist[4];
+                cellularDistances<D>( cp, pos, dist );
+                R[in.r[0]] = scaleOutput( cellularReturn( dist, in.p[0], in.p[1], in.p[2], in.p[3], 0.0f ), K[1], K[2], K[4] );
+                return;
+            }
+        if constexpr( opIn<OP, fn2::OP_CELL_LOOKUP>() )
         {
             float cell[D];
             cellularLookupPos<D>( cp, pos, cell );
@@ -162,15 +160,18 @@ namespace fn2i
         }
     }
 
-    template<int D>
+    template<int D, int OP = -1>
     FN2D void warpOp( const Ins& in, const float* K, int32_t seed, float* R )
     {
         float sp[D], out[D], axis[D];
 #pragma unroll
         for( int a = 0; a < D; a++ ) { sp[a] = R[in.r[1] + a]; out[a] = R[in.r[2] + a]; axis[a] = K[a]; }
         const float amp = R[in.r[3]];
-        float s;
-        if( in.op == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );
+        float s = 0.0f;
+        if constexpr( OP == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );
+        else if constexpr( OP == fn2::OP_WARP_SIMPLEX ) s = warpSimplex<D>( seed, in.p[0], amp, axis, sp, out );
+        else if constexpr( OP == fn2::OP_WARP_SUPERSIMPLEX ) s = warpSuperSimplex<D>( seed, in.p[0], amp, axis, sp, out );
+        else if( in.op == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );
         else if( in.op == fn2::OP_WARP_SIMPLEX ) s = warpSimplex<D>( seed, in.p[0], amp, axis, sp, out );
         else s = warpSuperSimplex<D>( seed, in.p[0], amp, axis, sp, out );
 #pragma unroll
@@ -195,7 +196,10 @@ namespace fn2i
     // (and with it occupancy) of lattice-only programs low.
     // One instruction. `K` points at the instruction's constants, `seed` is
     // the call seed plus the instruction's seed delta.
-    template<int DM, int FT>
+    // OP / LEAF: -1 in the interpreter; a specialised kernel passes the
+    // instruction's op (and, for OP_FRACTAL_LEAF, its leaf op) so that only
+    // their code is instantiated.
+    template<int DM, int FT, int OP = -1, int LEAF = -1>
     FN2D void execIns( const Ins& in, const float* K, const int32_t seed, float* R )
     {
         {
@@ -203,41 +207,43 @@ namespace fn2i
             switch( in.op )
             {
             case fn2::OP_PERLIN: case fn2::OP_VALUE: case fn2::OP_SIMPLEX: case fn2::OP_SUPERSIMPLEX:
-            {
-                float v = 0.0f;
-                if( ( DM & 4 ) && d == 2 ) leafAt<2>( in, K, seed, R, in.r[1], v );
-                else if( ( DM & 8 ) && d == 3 ) leafAt<3>( in, K, seed, R, in.r[1], v );
-                else if( DM & 16 ) leafAt<4>( in, K, seed, R, in.r[1], v );
-                R[in.r[0]] = v;
-                break;
-            }
-            case fn2::OP_FRACTAL_LEAF:
-            {
-                float v = 0.0f;
-                if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2>( in, K, seed, R, in.r[1] );
-                else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3>( in, K, seed, R, in.r[1] );
-                else if( DM & 16 ) v = fractalLeaf<4>( in, K, seed, R, in.r[1] );
-                R[in.r[0]] = v;
-                break;
-            }
-            case fn2::OP_CELL_VALUE: case fn2::OP_CELL_DISTANCE: case fn2::OP_CELL_LOOKUP:
-                if constexpr( ( FT & fn2::kFeatCell ) != 0 )
+                if constexpr( opIn<OP, fn2::OP_PERLIN, fn2::OP_VALUE, fn2::OP_SIMPLEX, fn2::OP_SUPERSIMPLEX>() )
                 {
-                    if( ( DM & 4 ) && d == 2 ) cellular<2>( in, K, seed, R );
-                    else if( ( DM & 8 ) && d == 3 ) cellular<3>( in, K, seed, R );
-                    else if( DM & 16 ) cellular<4>( in, K, seed, R );
+                    float v = 0.0f;
+                    if( ( DM & 4 ) && d == 2 ) leafAt<2, OP>( in, K, seed, R, in.r[1], v );
+                    else if( ( DM & 8 ) && d == 3 ) leafAt<3, OP>( in, K, seed, R, in.r[1], v );
+                    else if( DM & 16 ) leafAt<4, OP>( in, K, seed, R, in.r[1], v );
+                    R[in.r[0]] = v;
+                }
+                break;
+            case fn2::OP_FRACTAL_LEAF:
+                if constexpr( opIn<OP, fn2::OP_FRACTAL_LEAF>() )
+                {
+                    float v = 0.0f;
+                    if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2, LEAF>( in, K, seed, R, in.r[1] );
+                    else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3, LEAF>( in, K, seed, R, in.r[1] );
+                    else if( DM & 16 ) v = fractalLeaf<4, LEAF>( in, K, seed, R, in.r[1] );
+                    R[in.r[0]] = v;
+                }
+                break;
+            case fn2::OP_CELL_VALUE: case fn2::OP_CELL_DISTANCE: case fn2::OP_CELL_LOOKUP:
+                if constexpr( ( FT & fn2::kFeatCell ) != 0 && opIn<OP, fn2::OP_CELL_VALUE, fn2::OP_CELL_DISTANCE, fn2::OP_CELL_LOOKUP>() )
+                {
+                    if( ( DM & 4 ) && d == 2 ) cellular<2, OP>( in, K, seed, R );
+                    else if( ( DM & 8 ) && d == 3 ) cellular<3, OP>( in, K, seed, R );
+                    else if( DM & 16 ) cellular<4, OP>( in, K, seed, R );
                 }
                 break;
             case fn2::OP_WARP_GRADIENT: case fn2::OP_WARP_SIMPLEX: case fn2::OP_WARP_SUPERSIMPLEX:
-                if constexpr( ( FT & fn2::kFeatWarp ) != 0 )
+                if constexpr( ( FT & fn2::kFeatWarp ) != 0 && opIn<OP, fn2::OP_WARP_GRADIENT, fn2::OP_WARP_SIMPLEX, fn2::OP_WARP_SUPERSIMPLEX>() )
                 {
-                    if( ( DM & 4 ) && d == 2 ) warpOp<2>( in, K, seed, R );
-                    else if( ( DM & 8 ) && d == 3 ) warpOp<3>( in, K, seed, R );
-                    else if( DM & 16 ) warpOp<4>( in, K, seed, R );
+                    if( ( DM & 4 ) && d == 2 ) warpOp<2, OP>( in, K, seed, R );
+                    else if( ( DM & 8 ) && d == 3 ) warpOp<3, OP>( in, K, seed, R );
+                    else if( DM & 16 ) warpOp<4, OP>( in, K, seed, R );
                 }
                 break;
             case fn2::OP_WHITE:
-            if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+            if constexpr( ( FT & fn2::kFeatMisc ) != 0 && opIn<OP, fn2::OP_WHITE>() )
             {
                 // White (BasicGenerators.inl:45-59): hash of the raw position bits
                 const int32_t prime[4] = { kPX, kPY, kPZ, kPW };
@@ -250,22 +256,25 @@ namespace fn2i
                 R[in.r[0]] = scaleOutput( valueCoordFinal( h ), K[1], K[2], K[3] );
                 break;
             }
+            break;
             case fn2::OP_CHECKERBOARD:
-            if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+            if constexpr( ( FT & fn2::kFeatMisc ) != 0 && opIn<OP, fn2::OP_CHECKERBOARD>() )
             {
                 int32_t v = roundToInt( R[in.r[1]] * K[0] );
                 for( int ax = 1; ax < d; ax++ ) v ^= roundToInt( R[in.r[1] + ax] * K[0] );
                 R[in.r[0]] = scaleOutput( ibits( ishl( v & 1, 30 ) ), K[1], K[2], K[3] );
                 break;
             }
+            break;
             case fn2::OP_SINEWAVE:
-            if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+            if constexpr( ( FT & fn2::kFeatMisc ) != 0 && opIn<OP, fn2::OP_SINEWAVE>() )
             {
                 float r = sinf( R[in.r[1] + d - 1] * K[0] );
                 for( int ax = d - 2; ax >= 0; ax-- ) r = sinf( R[in.r[1] + ax] * K[0] ) * r;
                 R[in.r[0]] = scaleOutput( r, K[1], K[2], K[3] );
                 break;
             }
+            break;
             case fn2::OP_CONST: R[in.r[0]] = K[0]; break;
             case fn2::OP_MOV: R[in.r[0]] = R[in.r[1]]; break;
             case fn2::OP_ADD: R[in.r[0]] = R[in.r[1]] + R[in.r[2]]; break;
@@ -456,7 +465,7 @@ namespace fn2i
                 R[in.r[0]] = fmadd( R[in.r[1]] + R[in.r[2]], K[0], R[in.r[0]] );
                 break;
             case fn2::OP_DISTANCE:
-                if constexpr( ( FT & fn2::kFeatMisc ) != 0 )
+                if constexpr( ( FT & fn2::kFeatMisc ) != 0 && opIn<OP, fn2::OP_DISTANCE>() )
                 {
                     if( ( DM & 4 ) && d == 2 ) distanceOp<2>( in, R );
                     else if( ( DM & 8 ) && d == 3 ) distanceOp<3>( in, R );

@@ -15,6 +15,7 @@
 // from fn2SetJitMode or the FN2_JIT environment variable (off / async / sync).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <algorithm>
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
@@ -51,7 +52,7 @@ namespace fn2
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         };
 
-        enum State { kQueued, kReady, kFailed };
+        enum State { kQueued, kReady, kFailed, kTooLarge };
 
         struct Entry
         {
@@ -60,6 +61,21 @@ namespace fn2
             std::string error;
         };
 
+        // The feature group an op needs compiled in (see fn2::kFeat*)
+        int OpFeature( int op )
+        {
+            switch( op )
+            {
+            case OP_CELL_VALUE: case OP_CELL_DISTANCE: case OP_CELL_LOOKUP: return kFeatCell;
+            case OP_WARP_GRADIENT: case OP_WARP_SIMPLEX: case OP_WARP_SUPERSIMPLEX: return kFeatWarp;
+            case OP_WHITE: case OP_CHECKERBOARD: case OP_SINEWAVE: case OP_DISTANCE: return kFeatMisc;
+            default: return 0;
+            }
+        }
+
+        // Each instruction becomes an `execIns` instantiated for its own op,
+        // dimensionality and (fused fractal) leaf op, so the compiler sees only
+        // the code that instruction runs.
         std::string Source( const DeviceProgram& p )
         {
             std::ostringstream s;
@@ -73,7 +89,9 @@ namespace fn2
                 for( int i = 0; i < 4; i++ ) s << (int)in.p[i] << ( i < 3 ? ", " : " }, { " );
                 for( int i = 0; i < 6; i++ ) s << (int)in.r[i] << ( i < 5 ? ", " : " }, " );
                 s << "(int32_t)" << (uint32_t)in.seed << "u, " << in.k << "u };\n"
-                  << "  execIns<28, 7>( in, a.konst + " << in.k << "u, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
+                  << "  execIns<" << ( in.dim >= 2 && in.dim <= 4 ? 1 << in.dim : 4 ) << ", " << OpFeature( in.op ) << ", "
+                  << (int)in.op << ", " << ( in.op == OP_FRACTAL_LEAF ? (int)in.p[0] : -1 )
+                  << ">( in, a.konst + " << in.k << "u, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
             }
             s << "return R[" << p.resultReg << "];\n} };\n}\n"
                  "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_single( fn2::GenArgs a )\n"
@@ -147,6 +165,8 @@ namespace fn2
                     if( !strcmp( env, "off" ) || !strcmp( env, "0" ) ) mode = FN2_JIT_OFF;
                     else if( !strcmp( env, "sync" ) ) mode = FN2_JIT_SYNC;
                 }
+                const char* lim = getenv( "FN2_JIT_MAX_INSTRUCTIONS" );
+                if( lim && atoi( lim ) > 0 ) maxInstructions = atoi( lim );
             }
 
             ~JitManager() { Shutdown(); }
@@ -165,6 +185,7 @@ namespace fn2
             }
 
             int mode = FN2_JIT_ASYNC;
+            size_t maxInstructions = 64;
 
             const JitKernel* Get( const DeviceProgram* p )
             {
@@ -174,6 +195,11 @@ namespace fn2
                 if( it == entries.end() )
                 {
                     it = entries.emplace( p, Entry() ).first;
+                    if( p->ins.size() > maxInstructions )
+                    {
+                        it->second.state = kTooLarge;
+                        return nullptr;
+                    }
                     if( mode == FN2_JIT_SYNC )
                     {
                         lk.unlock();
@@ -320,6 +346,31 @@ FASTNOISE_API int fn2JitCompileCheck( const void* node, int dims, int* codeBytes
     }
     if( codeBytes ) *codeBytes = (int)code.size();
     return 0;
+}
+
+FASTNOISE_API int fn2JitSource( const void* node, int dims, char* buf, int bufSize )
+{
+    fn2::ClearError();
+    if( !node ) { fn2::SetError( "null node" ); return -1; }
+    fn2::Program prog;
+    std::string err;
+    if( !fn2::CompileTree( static_cast<const FastNoise::Generator*>( node ), dims, prog, err ) )
+    {
+        fn2::SetError( "cannot compile node tree: " + err );
+        return -1;
+    }
+    fn2::DeviceProgram dp;
+    dp.ins = prog.ins;
+    dp.konst = prog.konst;
+    dp.resultReg = prog.resultReg;
+    std::string src = fn2::Source( dp );
+    if( buf && bufSize > 0 )
+    {
+        size_t n = std::min( src.size(), (size_t)bufSize - 1 );
+        memcpy( buf, src.data(), n );
+        buf[n] = 0;
+    }
+    return (int)src.size();
 }
 
 FASTNOISE_API int fn2GetJitState( const void* node, int dims )

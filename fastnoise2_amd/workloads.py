@@ -176,3 +176,37 @@ def grid_spec_like(dims, points=1 << 14, seed=3):
     from . import grid_spec
     side = max(2, int(round(points ** (1.0 / dims))))
     return grid_spec(dims, [0.0] * dims, [side] * dims, seed=seed)
+
+
+def bench_workloads():
+    """The single-tree throughput workloads: name -> tree, call shape and the
+    algorithmic work per point.
+
+    `ops` is (FP32 FLOPs with an fma counted as 2, vector ALU operations) per
+    point, counted by the op-counting build of the oracle over `ops_sample`
+    (a corner of the workload's own grid, or a coarse grid over the position
+    cube); the counts are committed here and recounted by the CPU tests."""
+    return {
+        "fbm_simplex_2d_grid_4096x4096": dict(
+            tree=fbm_simplex_2d, kind="grid", dims=2, offset=[-2048, -2048], count=[4096, 4096], seed=42,
+            ops=(299.0, 572.0), ops_sample=dict(dims=2, offset=[-2048, -2048], count=[32, 32])),
+        "fbm_simplex_tileable_1024x1024": dict(
+            tree=fbm_simplex_2d, kind="tileable", dims=2, count=[1024, 1024], seed=42,
+            ops=(653.0, 1703.0), ops_sample=dict(dims=2, offset=None, count=[64, 64], tileable=True)),
+        "perlin_3d_grid_64x64x64": dict(
+            tree=perlin_3d, kind="grid", dims=3, offset=[0, 0, 0], count=[64, 64, 64], seed=1337,
+            ops=(69.0, 209.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
+        "warped_cellular_3d_grid_256x256x256": dict(
+            tree=warped_cellular_3d, kind="grid", dims=3, offset=[0, 0, 0], count=[256, 256, 256], seed=1337,
+            ops=(837.0, 1229.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
+        "warped_cellular_positions_16777216": dict(
+            tree=warped_cellular_3d, kind="positions", dims=3, count=[1 << 24], seed=1337,
+            ops=(837.0, 1229.0),
+            ops_sample=dict(dims=3, offset=[-1000, -1000, -1000], count=[16, 16, 16], step=[125, 125, 125])),
+        "serving_tree_3d_grid_256x256x256": dict(
+            tree=serving_tree, kind="grid", dims=3, offset=[0, 0, 0], count=[256, 256, 256], seed=1337,
+            ops=(1950.0, 4255.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
+        "mixed_tree_deepest_3d_grid_128x128x128": dict(
+            tree=lambda: decode(mixed_trees()[-1]), kind="grid", dims=3, offset=[0, 0, 0], count=[128, 128, 128],
+            seed=1337, ops=(24583.0, 41557.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
+    }

@@ -73,7 +73,9 @@ FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n 
  * results, no interpretation). OFF: interpreter only. ASYNC (default):
  * compile in the background on first use, interpreter meanwhile. SYNC:
  * compile on first use. The FN2_JIT environment variable (off / async /
- * sync) sets the initial mode. fn2SetJitMode returns the previous mode. */
+ * sync) sets the initial mode. fn2SetJitMode returns the previous mode.
+ * Programs longer than FN2_JIT_MAX_INSTRUCTIONS (environment, default 64)
+ * stay on the interpreter: their compile time grows to tens of seconds. */
 enum fn2JitMode
 {
     FN2_JIT_OFF = 0,
@@ -85,11 +87,16 @@ FASTNOISE_API int fn2SetJitMode( int mode );
  * without loading it: 0 on success (codeBytes = size of the code object),
  * nonzero with the compiler log in fn2GetLastError otherwise. */
 FASTNOISE_API int fn2JitCompileCheck( const void* node, int dims, int* codeBytes );
+/* Copies the specialised kernel's HIP source for a tree into buf (at most
+ * bufSize bytes, NUL-terminated) and returns the source's full length, or -1
+ * when the tree does not compile. */
+FASTNOISE_API int fn2JitSource( const void* node, int dims, char* buf, int bufSize );
 /* Waits until every queued specialisation has finished compiling. */
 FASTNOISE_API int fn2JitWait( void );
 /* State of a tree's specialised kernel in `dims` dimensions: -1 never
  * requested, 0 queued, 1 ready, 2 failed (fn2GetLastError holds the
- * compiler log), -2 bad arguments. */
+ * compiler log), 3 kept on the interpreter (over the instruction limit),
+ * -2 bad arguments. */
 FASTNOISE_API int fn2GetJitState( const void* node, int dims );
 
 /* Compiled-program facts for a tree evaluated in `dims` dimensions. */

@@ -69,3 +69,29 @@ def test_specialised_batches_in_serving(gpu, jit_sync):
         assert_bitexact(out, ref, "specialised batch")
         assert mm == ref_mm
     eng.close()
+
+
+# every fifth node case: all node families, dimensions mixed
+from cases import CASES, REGION  # noqa: E402
+
+JIT_CASES = CASES[::5]
+
+
+@pytest.mark.parametrize("name,spec,dims,tol,seed", JIT_CASES, ids=[c[0] for c in JIT_CASES])
+def test_node_cases_specialised_equal_interpreter(gpu, name, spec, dims, tol, seed):
+    node = fn.Node.from_encoded(spec.encode())
+    r = REGION[dims]
+    gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d, 4: node.gen_uniform_grid_4d}[dims]
+    args = list(r["offset"]) + list(r["count"]) + list(r["step"])
+    old = fn.set_jit_mode(fn.JIT_OFF)
+    try:
+        interp, imm = gen(*args, seed=seed)
+        fn.set_jit_mode(fn.JIT_SYNC)
+        spec_out, smm = gen(*args, seed=seed)
+        state = node.jit_state(dims)
+    finally:
+        fn.set_jit_mode(old)
+    assert state in (1, 3), fn.last_error()
+    # same device arithmetic on both paths: identical bits, NaNs included
+    assert_bitexact(spec_out, interp, name)
+    assert imm == smm or (np.isnan(imm).all() and np.isnan(smm).all())

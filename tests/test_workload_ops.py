@@ -1,0 +1,17 @@
+"""The work-per-point figures the throughput bench prices its roofline with
+are the op-counting oracle's counts for each workload's tree."""
+import pytest
+
+from fastnoise2_amd import workloads as W
+import oracle
+
+WL = W.bench_workloads()
+
+
+@pytest.mark.parametrize("name", sorted(WL))
+def test_committed_op_counts_match_the_counting_oracle(name):
+    wl = WL[name]
+    s = dict(wl["ops_sample"])
+    flop, valu = oracle.op_counts_per_point(wl["tree"]().encode(), s.pop("dims"), s.pop("offset"), s.pop("count"),
+                                            seed=wl["seed"], **s)
+    assert (flop, valu) == pytest.approx(wl["ops"], rel=1e-6), (flop, valu)
