@@ -159,7 +159,7 @@ namespace fn2
         auto& bucket = gPrograms[p.hash];
         for( DeviceProgram* d : bucket )
         {
-            if( d->resultReg == p.resultReg && d->ins.size() == p.ins.size() && d->konst.size() == p.konst.size() &&
+            if( d->resultReg == p.resultReg && d->dims == p.dims && d->ins.size() == p.ins.size() && d->konst.size() == p.konst.size() &&
                 memcmp( d->ins.data(), p.ins.data(), p.ins.size() * sizeof( Ins ) ) == 0 &&
                 memcmp( d->konst.data(), p.konst.data(), p.konst.size() * sizeof( float ) ) == 0 )
             {
@@ -205,6 +205,7 @@ namespace fn2
         d->resultReg = p.resultReg;
         d->dimMask = p.dimMask;
         d->featMask = p.featMask;
+        d->dims = p.dims;
         size_t insBytes = std::max<size_t>( 1, p.ins.size() ) * sizeof( Ins );
         // padded so that a window starting at the last constant stays in bounds
         size_t kBytes = ( p.konst.size() + 16 ) * sizeof( float );

@@ -20,6 +20,7 @@ namespace fn2
         int resultReg = 0;
         int dimMask = 0;
         int featMask = 0;
+        int dims = 0;    // of the starting positions (programs are cached per dimensionality)
         Ins* dIns = nullptr;
         float* dKonst = nullptr;
     };

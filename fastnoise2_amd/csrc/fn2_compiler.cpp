@@ -593,6 +593,8 @@ namespace fn2
         h = Fnv( out.ins.data(), out.ins.size() * sizeof( Ins ), h );
         h = Fnv( out.konst.data(), out.konst.size() * sizeof( float ), h );
         h = Fnv( &out.resultReg, sizeof( int ), h );
+        out.dims = dims;
+        h = Fnv( &out.dims, sizeof( int ), h );
         out.hash = h;
         return true;
     }

@@ -17,7 +17,8 @@ namespace fn2
         int dimMask = 0;     // bit d set: d-dimensional ops used
         int featMask = 0;    // kFeatCell | kFeatWarp | kFeatMisc
         int regsUsed = 0;
-        uint64_t hash = 0;   // of ins + konst, the device cache key
+        int dims = 0;        // dimensionality of the starting positions
+        uint64_t hash = 0;   // of ins + konst + dims, the device cache key
     };
 
     // Compiles `root` for evaluation at `dims`-dimensional positions that the

@@ -489,6 +489,8 @@ namespace fn2i
         return R[a.resultReg];
     }
 
+    // Starting position of point `idx` into R[0..dims): grid, position arrays
+    // or the tileable torus (Generator.inl:79-433).
     FN2D void startPosition( const fn2::GenArgs& a, uint32_t idx, float* R )
     {
         if( a.kind == fn2::SRC_POSITIONS )
@@ -516,5 +518,43 @@ namespace fn2i
             }
             R[a.dims - 1] = fmadd( (float)(int32_t)rem, a.step[a.dims - 1], a.offset[a.dims - 1] );
         }
+    }
+
+    // The same for a program compiled for D-dimensional positions (specialised
+    // kernels): every source writes the same D registers, from values computed
+    // per source, so the register file never needs a run-time index. A
+    // tileable call always runs a 4D program.
+    template<int D>
+    FN2D void startPositionD( const fn2::GenArgs& a, uint32_t idx, float* R )
+    {
+        float p[D];
+        if( a.kind == fn2::SRC_POSITIONS )
+        {
+#pragma unroll
+            for( int ax = 0; ax < D; ax++ ) p[ax] = a.offset[ax] + a.pos[ax][idx];
+        }
+        else if( D == 4 && a.kind == fn2::SRC_TILEABLE )
+        {
+            uint32_t yi = fastDiv( a.div[0], idx );
+            uint32_t xi = idx - yi * a.div[0].d;
+            p[0] = a.pos[0][xi];
+            p[1] = a.pos[1][yi];
+            p[D > 2 ? 2 : 0] = a.pos[2][xi];
+            p[D > 3 ? 3 : 0] = a.pos[3][yi];
+        }
+        else
+        {
+            uint32_t rem = idx;
+#pragma unroll
+            for( int ax = 0; ax < D - 1; ax++ )
+            {
+                uint32_t q = fastDiv( a.div[ax], rem );
+                p[ax] = fmadd( (float)(int32_t)( rem - q * a.div[ax].d ), a.step[ax], a.offset[ax] );
+                rem = q;
+            }
+            p[D - 1] = fmadd( (float)(int32_t)rem, a.step[D - 1], a.offset[D - 1] );
+        }
+#pragma unroll
+        for( int ax = 0; ax < D; ax++ ) R[ax] = p[ax];
     }
 }

@@ -61,6 +61,10 @@ namespace fn2
             std::string error;
         };
 
+        // Constants one instruction may read past its pool index (the device
+        // pool carries the same padding)
+        constexpr int kJitConstWindow = 16;
+
         // The feature group an op needs compiled in (see fn2::kFeat*)
         int OpFeature( int op )
         {
@@ -82,16 +86,26 @@ namespace fn2
             s << "#include \"fn2_kernel_body.h\"\n"
                  "using namespace fn2d;\nusing namespace fn2i;\n"
                  "namespace {\nstruct Prog {\n"
+                 "static constexpr bool kFresh = true;\n"
+                 "static constexpr int kDims = " << p.dims << ";\n"
                  "__device__ __forceinline__ float operator()( const fn2::GenArgs& a, float* R ) const {\n";
             for( const Ins& in : p.ins )
             {
                 s << "{ constexpr fn2::Ins in = { " << (int)in.op << ", " << (int)in.dim << ", { ";
                 for( int i = 0; i < 4; i++ ) s << (int)in.p[i] << ( i < 3 ? ", " : " }, { " );
                 for( int i = 0; i < 6; i++ ) s << (int)in.r[i] << ( i < 5 ? ", " : " }, " );
-                s << "(int32_t)" << (uint32_t)in.seed << "u, " << in.k << "u };\n"
-                  << "  execIns<" << ( in.dim >= 2 && in.dim <= 4 ? 1 << in.dim : 4 ) << ", " << OpFeature( in.op ) << ", "
+                s << "(int32_t)" << (uint32_t)in.seed << "u, " << in.k << "u };\n";
+                // the instruction's constant window as literals (exact bit patterns)
+                s << "  constexpr float k[" << kJitConstWindow << "] = { ";
+                for( int i = 0; i < kJitConstWindow; i++ )
+                {
+                    uint32_t bits = 0;
+                    if( in.k + i < p.konst.size() ) memcpy( &bits, &p.konst[in.k + i], 4 );
+                    s << "__builtin_bit_cast( float, 0x" << std::hex << bits << std::dec << "u )" << ( i + 1 < kJitConstWindow ? ", " : " };\n" );
+                }
+                s << "  execIns<" << ( in.dim >= 2 && in.dim <= 4 ? 1 << in.dim : 4 ) << ", " << OpFeature( in.op ) << ", "
                   << (int)in.op << ", " << ( in.op == OP_FRACTAL_LEAF ? (int)in.p[0] : -1 )
-                  << ">( in, a.konst + " << in.k << "u, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
+                  << ">( in, k, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
             }
             s << "return R[" << p.resultReg << "];\n} };\n}\n"
                  "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_single( fn2::GenArgs a )\n"
@@ -338,6 +352,7 @@ FASTNOISE_API int fn2JitCompileCheck( const void* node, int dims, int* codeBytes
     dp.ins = prog.ins;
     dp.konst = prog.konst;
     dp.resultReg = prog.resultReg;
+    dp.dims = prog.dims;
     std::vector<char> code;
     if( !fn2::CompileCode( dp, code, err ) )
     {
@@ -363,6 +378,7 @@ FASTNOISE_API int fn2JitSource( const void* node, int dims, char* buf, int bufSi
     dp.ins = prog.ins;
     dp.konst = prog.konst;
     dp.resultReg = prog.resultReg;
+    dp.dims = prog.dims;
     std::string src = fn2::Source( dp );
     if( buf && bufSize > 0 )
     {

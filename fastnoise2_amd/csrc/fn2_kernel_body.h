@@ -50,20 +50,46 @@ namespace fn2k
         }
     }
 
+    // The register file of one point. Interpreted programs index it at run
+    // time, so it is zeroed once per thread; a specialised program (whose
+    // every register is written before it is read) gets a fresh, unset file
+    // per point, which leaves the compiler no values to carry between points.
+    template<class Prog>
+    struct RegFile
+    {
+        float R[fn2::kMaxRegs];
+        __device__ __forceinline__ RegFile()
+        {
+            if constexpr( !Prog::kFresh )
+            {
+#pragma unroll
+                for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
+            }
+        }
+    };
+
     // One request: grid-stride loop over all of its points.
     template<int BLOCK, class Prog>
     __device__ __forceinline__ void generateSingle( const fn2::GenArgs& a, Prog prog )
     {
-        float R[fn2::kMaxRegs];
-#pragma unroll
-        for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
+        RegFile<Prog> file;
         float mn = __builtin_inff(), mx = -__builtin_inff();
         const uint32_t total = (uint32_t)a.total;
         const uint32_t stride = gridDim.x * BLOCK;
         for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
         {
-            startPosition( a, i, R );
-            const float v = prog( a, R );
+            float v;
+            if constexpr( Prog::kFresh )
+            {
+                RegFile<Prog> fresh;
+                startPositionD<Prog::kDims>( a, i, fresh.R );
+                v = prog( a, fresh.R );
+            }
+            else
+            {
+                startPosition( a, i, file.R );
+                v = prog( a, file.R );
+            }
             a.out[i] = v;
             if( v == v )
             {
@@ -84,9 +110,7 @@ namespace fn2k
         // the job is read through a pointer (uniform address -> scalar loads)
         // rather than copied into a private struct
         const fn2::GenArgs& a = jobs[blockJob[blockIdx.x]];
-        float R[fn2::kMaxRegs];
-#pragma unroll
-        for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
+        RegFile<Prog> file;
         float mn = __builtin_inff(), mx = -__builtin_inff();
         const uint32_t total = (uint32_t)a.total;
         const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
@@ -97,8 +121,18 @@ namespace fn2k
             const uint32_t i = base + p * BLOCK;
             if( i < total )
             {
-                startPosition( a, i, R );
-                const float v = prog( a, R );
+                float v;
+                if constexpr( Prog::kFresh )
+                {
+                    RegFile<Prog> fresh;
+                    startPositionD<Prog::kDims>( a, i, fresh.R );
+                    v = prog( a, fresh.R );
+                }
+                else
+                {
+                    startPosition( a, i, file.R );
+                    v = prog( a, file.R );
+                }
                 a.out[i] = v;
                 if( v == v )
                 {

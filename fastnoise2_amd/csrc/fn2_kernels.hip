@@ -21,11 +21,20 @@
 using namespace fn2d;
 using namespace fn2i;
 
+// The interpreter as a program: runs the request's instruction list.
+template<int DM, int FT>
+struct Interpreted
+{
+    static constexpr bool kFresh = false;   // indexes the register file at run time
+    static constexpr int kDims = 0;         // a.dims at run time
+    __device__ __forceinline__ float operator()( const fn2::GenArgs& g, float* R ) const { return runProgram<DM, FT>( g, R ); }
+};
+
 // Single call of the interpreter: grid-stride loop over all points of one request.
 template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate( fn2::GenArgs a )
 {
-    fn2k::generateSingle<BLOCK>( a, []( const fn2::GenArgs& g, float* R ) { return runProgram<DM, FT>( g, R ); } );
+    fn2k::generateSingle<BLOCK>( a, Interpreted<DM, FT>() );
 }
 
 // Batched form for the serving layer: every block looks up its request in a
@@ -34,7 +43,7 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate( fn2::GenArgs a )
 template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ blockJob, const uint32_t* __restrict__ blockBase )
 {
-    fn2k::generateBatch<BLOCK>( jobs, blockJob, blockBase, []( const fn2::GenArgs& g, float* R ) { return runProgram<DM, FT>( g, R ); } );
+    fn2k::generateBatch<BLOCK>( jobs, blockJob, blockBase, Interpreted<DM, FT>() );
 }
 
 __global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
