@@ -104,11 +104,12 @@ def progress(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
+def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=20000):
     """Chunk latency and throughput of the serving engine on the serving tree:
     the Poisson trace (open loop) for p50/p99 latency, and a saturating closed
-    loop (512 outstanding) of the same request mix for throughput, in the
-    plain (one launch + copy per chunk), batched and graph-replay modes."""
+    loop (512 outstanding) of the same request mix for throughput, into pinned
+    host buffers and into device buffers, in the plain (one launch + copy per
+    chunk), batched and graph-replay modes."""
     import fastnoise2_amd as fn
     from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
     enc = W.serving_tree().encode()
@@ -133,10 +134,18 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=4000):
                     "saturated_p50_ms": round(float(np.percentile(lat, 50)), 3),
                     "saturated_p99_ms": round(float(np.percentile(lat, 99)), 3),
                     "batches": eng.stats()["batches"]})
+        # the same closed loop with outputs left in device memory (consumers on
+        # the GPU): no PCIe copy, so launch and batching costs show
+        lat, wall = eng.replay([tid] * n, specs[:n], None, 512, device_outputs=True)
+        rec.update({"saturated_device_Gpoints_s": round(float(pts[:n].sum()) / wall / 1e9, 3),
+                    "saturated_device_p50_ms": round(float(np.percentile(lat, 50)), 3),
+                    "saturated_device_p99_ms": round(float(np.percentile(lat, 99)), 3)})
         eng.close()
         out[name] = rec
         progress("serving %s done" % name)
     out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
+    out["batched_vs_plain_device_throughput"] = round(out["batched"]["saturated_device_Gpoints_s"]
+                                                      / out["plain"]["saturated_device_Gpoints_s"], 2)
     return out
 
 

@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import GenSpec, SOURCE_GRID, SOURCE_POSITIONS, SOURCE_TILEABLE, last_error
 from .tree import NodeSpec, meta, registry, b64encode, b64decode
 
-__all__ = ["Node", "NodeSpec", "DeviceArray", "GenSpec", "lib", "last_error", "meta", "registry",
+__all__ = ["Node", "NodeSpec", "DeviceArray", "PinnedArray", "GenSpec", "lib", "last_error", "meta", "registry",
            "b64encode", "b64decode", "FEATURE_SET_GPU", "device_count"]
 
 FEATURE_SET_GPU = 0x1000
@@ -80,6 +80,31 @@ class DeviceArray:
     def free(self):
         if self.ptr:
             lib().fn2Free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A float32 numpy view of pinned host memory (fn2MallocHost), which the
+    device fills by DMA with no staging copy."""
+
+    def __init__(self, count):
+        self.count = int(count)
+        self.ptr = lib().fn2MallocHost(max(1, self.count) * 4)
+        if not self.ptr:
+            raise RuntimeError("fn2MallocHost failed: " + last_error())
+        buf = (ctypes.c_float * max(1, self.count)).from_address(self.ptr)
+        self.array = np.ctypeslib.as_array(buf)[:self.count]
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().fn2FreeHost(self.ptr)
             self.ptr = None
 
     def __del__(self):

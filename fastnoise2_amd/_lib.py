@@ -94,6 +94,8 @@ GPU_EXT = {
     "fn2BenchGenerate": (i, [vp, ctypes.POINTER(GenSpec), vp, i, i, c_double_p, c_double_p]),
     "fn2Malloc": (vp, [ctypes.c_size_t]),
     "fn2Free": (None, [vp]),
+    "fn2MallocHost": (vp, [ctypes.c_size_t]),
+    "fn2FreeHost": (None, [vp]),
     "fn2Memcpy": (i, [vp, vp, ctypes.c_size_t, i]),
     "fn2DeviceSynchronize": (i, []),
     "fn2DeviceCount": (i, []),
@@ -113,7 +115,7 @@ GPU_EXT = {
     "fn2ServeDrain": (i, [vp]),
     "fn2ServeStats": (i, [vp, c_int64_p, c_int64_p, c_int64_p]),
     "fn2ServeTreeTuning": (i, [vp, i, i]),
-    "fn2ServeReplay": (i, [vp, i, c_int_p, ctypes.POINTER(GenSpec), c_double_p, i, c_double_p, c_double_p]),
+    "fn2ServeReplay": (i, [vp, i, c_int_p, ctypes.POINTER(GenSpec), c_double_p, i, i, c_double_p, c_double_p]),
 }
 
 _lib = None

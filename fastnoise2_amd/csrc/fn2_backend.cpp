@@ -674,6 +674,20 @@ FASTNOISE_API void fn2Free( void* p )
     if( p ) (void)hipFree( p );
 }
 
+FASTNOISE_API void* fn2MallocHost( size_t bytes )
+{
+    ClearError();
+    if( !EnsureDevice() ) return nullptr;
+    void* p = nullptr;
+    if( !HipOk( hipHostMalloc( &p, bytes, hipHostMallocDefault ), "hipHostMalloc" ) ) return nullptr;
+    return p;
+}
+
+FASTNOISE_API void fn2FreeHost( void* p )
+{
+    if( p ) (void)hipHostFree( p );
+}
+
 FASTNOISE_API int fn2Memcpy( void* dst, const void* src, size_t bytes, int kind )
 {
     ClearError();

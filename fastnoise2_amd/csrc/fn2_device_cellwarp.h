@@ -146,8 +146,11 @@ namespace fn2d
         return (float)sel;
     }
 
-    // CellularDistance (Cellular.inl:294-529): the four smallest distances
-    template<int D>
+    // CellularDistance (Cellular.inl:294-529): the four smallest distances.
+    // Slot k of the insertion network depends only on slots below k, so a
+    // caller that reads slots < SLOTS only (distance indices known when the
+    // kernel is compiled) may keep just those: they hold the same values.
+    template<int D, int SLOTS = 4>
     FN2D void cellularDistances( const CellParams& cp, const float* pos, float* dist )
     {
         dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
@@ -157,9 +160,9 @@ namespace fn2d
             for( int a = 0; a < D; a++ ) d[a] = fmadd( v[a], invMag, fc[a] - pos[a] );
             float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );
             if( cp.sizeJitterOn ) nd *= fnmadd( cp.sizeJitter, sizeJitterBits<D>( h ), 1.0f );
-            dist[3] = fmaxf_( fminf_( dist[3], nd ), dist[2] );
-            dist[2] = fmaxf_( fminf_( dist[2], nd ), dist[1] );
-            dist[1] = fmaxf_( fminf_( dist[1], nd ), dist[0] );
+            if constexpr( SLOTS > 3 ) dist[3] = fmaxf_( fminf_( dist[3], nd ), dist[2] );
+            if constexpr( SLOTS > 2 ) dist[2] = fmaxf_( fminf_( dist[2], nd ), dist[1] );
+            if constexpr( SLOTS > 1 ) dist[1] = fmaxf_( fminf_( dist[1], nd ), dist[0] );
             dist[0] = fminf_( dist[0], nd );
         } );
     }

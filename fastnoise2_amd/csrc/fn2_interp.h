@@ -121,7 +121,9 @@ namespace fn2i
         return sum;
     }
 
-    template<int D, int OP = -1>
+    // SLOTS: distance slots CellularDistance keeps (4, or fewer when the
+    // instruction's distance indices are compile-time constants)
+    template<int D, int OP = -1, int SLOTS = 4>
     FN2D void cellular( const Ins& in, const float* K, int32_t seed, float* R )
     {
         CellParams cp;
@@ -146,7 +148,7 @@ namespace fn2i
             if( in.op == fn2::OP_CELL_DISTANCE )
             {
                 float dist[4];
-                cellularDistances<D>( cp, pos, dist );
+                cellularDistances<D, SLOTS>( cp, pos, dist );
                 R[in.r[0]] = scaleOutput( cellularReturn( dist, in.p[0], in.p[1], in.p[2], in.p[3], 0.0f ), K[1], K[2], K[4] );
                 return;
             }
@@ -196,10 +198,11 @@ namespace fn2i
     // (and with it occupancy) of lattice-only programs low.
     // One instruction. `K` points at the instruction's constants, `seed` is
     // the call seed plus the instruction's seed delta.
-    // OP / LEAF: -1 in the interpreter; a specialised kernel passes the
-    // instruction's op (and, for OP_FRACTAL_LEAF, its leaf op) so that only
-    // their code is instantiated.
-    template<int DM, int FT, int OP = -1, int LEAF = -1>
+    // OP / AUX: -1 in the interpreter; a specialised kernel passes the
+    // instruction's op and, for OP_FRACTAL_LEAF, its leaf op, for
+    // OP_CELL_DISTANCE the number of distance slots it reads, so that only
+    // the code the instruction runs is instantiated.
+    template<int DM, int FT, int OP = -1, int AUX = -1>
     FN2D void execIns( const Ins& in, const float* K, const int32_t seed, float* R )
     {
         {
@@ -220,18 +223,19 @@ namespace fn2i
                 if constexpr( opIn<OP, fn2::OP_FRACTAL_LEAF>() )
                 {
                     float v = 0.0f;
-                    if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2, LEAF>( in, K, seed, R, in.r[1] );
-                    else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3, LEAF>( in, K, seed, R, in.r[1] );
-                    else if( DM & 16 ) v = fractalLeaf<4, LEAF>( in, K, seed, R, in.r[1] );
+                    if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2, AUX>( in, K, seed, R, in.r[1] );
+                    else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3, AUX>( in, K, seed, R, in.r[1] );
+                    else if( DM & 16 ) v = fractalLeaf<4, AUX>( in, K, seed, R, in.r[1] );
                     R[in.r[0]] = v;
                 }
                 break;
             case fn2::OP_CELL_VALUE: case fn2::OP_CELL_DISTANCE: case fn2::OP_CELL_LOOKUP:
                 if constexpr( ( FT & fn2::kFeatCell ) != 0 && opIn<OP, fn2::OP_CELL_VALUE, fn2::OP_CELL_DISTANCE, fn2::OP_CELL_LOOKUP>() )
                 {
-                    if( ( DM & 4 ) && d == 2 ) cellular<2, OP>( in, K, seed, R );
-                    else if( ( DM & 8 ) && d == 3 ) cellular<3, OP>( in, K, seed, R );
-                    else if( DM & 16 ) cellular<4, OP>( in, K, seed, R );
+                    constexpr int slots = OP == fn2::OP_CELL_DISTANCE && AUX > 0 ? AUX : 4;
+                    if( ( DM & 4 ) && d == 2 ) cellular<2, OP, slots>( in, K, seed, R );
+                    else if( ( DM & 8 ) && d == 3 ) cellular<3, OP, slots>( in, K, seed, R );
+                    else if( DM & 16 ) cellular<4, OP, slots>( in, K, seed, R );
                 }
                 break;
             case fn2::OP_WARP_GRADIENT: case fn2::OP_WARP_SIMPLEX: case fn2::OP_WARP_SUPERSIMPLEX:

@@ -77,6 +77,15 @@ namespace fn2
             }
         }
 
+        // Compile-time detail of an instruction beyond its op: the leaf op of a
+        // fused fractal, the distance slots a CellularDistance reads
+        int Aux( const Ins& in )
+        {
+            if( in.op == OP_FRACTAL_LEAF ) return in.p[0];
+            if( in.op == OP_CELL_DISTANCE ) return std::min( 3, std::max<int>( in.p[1], in.p[2] ) ) + 1;
+            return -1;
+        }
+
         // Each instruction becomes an `execIns` instantiated for its own op,
         // dimensionality and (fused fractal) leaf op, so the compiler sees only
         // the code that instruction runs.
@@ -104,7 +113,7 @@ namespace fn2
                     s << "__builtin_bit_cast( float, 0x" << std::hex << bits << std::dec << "u )" << ( i + 1 < kJitConstWindow ? ", " : " };\n" );
                 }
                 s << "  execIns<" << ( in.dim >= 2 && in.dim <= 4 ? 1 << in.dim : 4 ) << ", " << OpFeature( in.op ) << ", "
-                  << (int)in.op << ", " << ( in.op == OP_FRACTAL_LEAF ? (int)in.p[0] : -1 )
+                  << (int)in.op << ", " << Aux( in )
                   << ">( in, k, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
             }
             s << "return R[" << p.resultReg << "];\n} };\n}\n"

@@ -85,7 +85,10 @@ namespace
         int job = 0;
         int ppt = 1;
         const fn2::JitKernel* jit = nullptr;
-        bool direct = false;   // hostOut is pinned: copied device -> caller directly
+        // where the values go: through the slot's pinned staging buffer (pageable
+        // caller memory), straight from the arena into pinned caller memory, or
+        // into caller device memory (a device-to-device copy)
+        enum Dest { kStaged, kPinned, kDevice } dest = kStaged;
         // results
         bool done = false;
         int status = 0;
@@ -354,12 +357,10 @@ namespace
             r->tree = treeId;
             r->spec = spec;
             r->hostOut = hostOut;
-            // pinned destinations receive their values straight from the device
-            // (not in graph mode, whose recorded copies must not depend on the caller)
-            if( mode != FN2_SERVE_GRAPH )
             {
                 hipPointerAttribute_t attr;
-                if( hipPointerGetAttributes( &attr, hostOut ) == hipSuccess ) r->direct = attr.type == hipMemoryTypeHost;
+                if( hipPointerGetAttributes( &attr, hostOut ) == hipSuccess )
+                    r->dest = attr.type == hipMemoryTypeHost ? Request::kPinned : attr.type == hipMemoryTypeDevice ? Request::kDevice : Request::kStaged;
                 else (void)hipGetLastError();
             }
             std::string err;
@@ -659,13 +660,15 @@ namespace
                 if( !fn2::HipOk( hipGraphLaunch( s.exec, s.stream ), "graph launch" ) ) return false;
             }
             else if( !Enqueue( s, shape, kernels, nJobs, nb ) ) return false;
+            if( !Copies( s, nJobs ) ) return false;
             if( !fn2::HipOk( hipEventRecord( s.done, s.stream ), "serve event" ) ) return false;
             s.reqs.swap( order );
             return true;
         }
 
-        // Stream work of one batch: table upload, range-key reset, one batch
-        // launch per kernel variant, copies of outputs and keys to pinned memory.
+        // Stream work of one batch that depends only on its shape (recorded
+        // once per shape in graph mode): table upload, range-key reset, one
+        // batch launch per kernel.
         bool Enqueue( Slot& s, const std::vector<std::pair<int64_t, int>>& shape, const std::vector<const fn2::JitKernel*>& kernels,
                       int nJobs, int nb )
         {
@@ -697,19 +700,24 @@ namespace
                 }
                 first += g.second;
             }
-            // the requests occupy [0, points) of the arena
-            size_t outBytes = (size_t)s.points * 4;
-            bool staged = false;
+            return true;
+        }
+
+        // Copies of one batch's results, issued on the slot's stream after its
+        // launches (never recorded: destinations change from batch to batch).
+        bool Copies( Slot& s, int nJobs )
+        {
+            size_t stagedEnd = 0;   // the staged requests lie in [0, stagedEnd) of the arena
             for( Request* r : s.reqs )
             {
-                if( r->direct )
-                {
-                    if( !fn2::HipOk( hipMemcpyAsync( r->hostOut, s.dOut + r->outOffset, (size_t)r->points * 4, hipMemcpyDeviceToHost, s.stream ), "copy output" ) )
-                        return false;
-                }
-                else staged = true;
+                const size_t bytes = (size_t)r->points * 4;
+                if( r->dest == Request::kStaged ) stagedEnd = std::max( stagedEnd, r->outOffset + (size_t)r->points );
+                else if( !fn2::HipOk( hipMemcpyAsync( r->hostOut, s.dOut + r->outOffset, bytes,
+                                                      r->dest == Request::kDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s.stream ),
+                                      "copy output" ) )
+                    return false;
             }
-            if( staged && !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, outBytes, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
+            if( stagedEnd && !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, stagedEnd * 4, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
             if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, 2 * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
             return true;
         }
@@ -764,7 +772,7 @@ namespace
                     std::vector<CopyPool::Seg> segs;
                     segs.reserve( s.reqs.size() );
                     for( Request* r : s.reqs )
-                        if( !r->direct ) segs.push_back( { r->hostOut, s.hOut + r->outOffset, (size_t)r->points } );
+                        if( r->dest == Request::kStaged ) segs.push_back( { r->hostOut, s.hOut + r->outOffset, (size_t)r->points } );
                     copies->Run( segs );
                     for( Request* r : s.reqs )
                     {
@@ -870,7 +878,7 @@ FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims )
 }
 
 FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,
-                                  int maxOutstanding, double* latencyMs, double* wallSec )
+                                  int maxOutstanding, int deviceOutputs, double* latencyMs, double* wallSec )
 {
     fn2::ClearError();
     if( !engine || count < 0 || !treeIds || !specs || !latencyMs || maxOutstanding < 1 )
@@ -893,16 +901,24 @@ FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, c
         }
         maxPoints = std::max( maxPoints, n );
     }
-    // pinned buffers, as a serving front end would hold: the engine copies into them directly
+    // pinned host buffers, as a serving front end would hold (the engine
+    // copies into them directly), or device buffers for consumers on the GPU
     std::vector<float*> pool( (size_t)maxOutstanding, nullptr );
     struct PoolFree
     {
         std::vector<float*>& p;
-        ~PoolFree() { for( float* b : p ) if( b ) (void)hipHostFree( b ); }
-    } poolFree{ pool };
+        bool device;
+        ~PoolFree()
+        {
+            for( float* b : p )
+                if( b ) (void)( device ? hipFree( b ) : hipHostFree( b ) );
+        }
+    } poolFree{ pool, deviceOutputs != 0 };
+    const size_t poolBytes = (size_t)std::max<int64_t>( maxPoints, 1 ) * 4;
     for( float*& b : pool )
     {
-        if( !fn2::HipOk( hipHostMalloc( (void**)&b, (size_t)std::max<int64_t>( maxPoints, 1 ) * 4, hipHostMallocDefault ), "replay buffers" ) ) return 1;
+        hipError_t err = deviceOutputs ? hipMalloc( (void**)&b, poolBytes ) : hipHostMalloc( (void**)&b, poolBytes, hipHostMallocDefault );
+        if( !fn2::HipOk( err, "replay buffers" ) ) return 1;
     }
     std::deque<std::pair<int64_t, int>> pending;   // (request id, pool slot), in submit order
     std::vector<int> freeSlots;

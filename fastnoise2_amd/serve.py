@@ -39,12 +39,19 @@ class ServeEngine:
         return t
 
     def submit(self, tree, spec, out=None):
-        """Queues one request; `out` (float32, spec's point count) receives the values."""
+        """Queues one request; `out` receives the values: a float32 numpy array
+        of the spec's point count (default: a new one), or a DeviceArray /
+        PinnedArray of at least that many floats."""
         n = spec_points(spec)
         if out is None:
             out = np.empty(n, dtype=np.float32)
-        assert out.dtype == np.float32 and out.size >= n and out.flags.c_contiguous
-        rid = self.lib.fn2ServeSubmit(self.h, tree, ctypes.byref(spec), out.ctypes.data_as(_lib.c_float_p))
+        if isinstance(out, np.ndarray):
+            assert out.dtype == np.float32 and out.size >= n and out.flags.c_contiguous
+            ptr = out.ctypes.data_as(_lib.c_float_p)
+        else:
+            assert out.count >= n
+            ptr = ctypes.cast(out.ptr, _lib.c_float_p)
+        rid = self.lib.fn2ServeSubmit(self.h, tree, ctypes.byref(spec), ptr)
         if rid < 0:
             raise RuntimeError("fn2ServeSubmit failed: " + last_error())
         self._outs[rid] = out
@@ -63,8 +70,9 @@ class ServeEngine:
     def drain(self):
         self.lib.fn2ServeDrain(self.h)
 
-    def replay(self, tree_ids, specs, arrivals=None, max_outstanding=64):
-        """Native trace replay (fn2ServeReplay): returns (latencies in ms, wall seconds)."""
+    def replay(self, tree_ids, specs, arrivals=None, max_outstanding=64, device_outputs=False):
+        """Native trace replay (fn2ServeReplay): returns (latencies in ms, wall seconds).
+        Outputs go to pinned host buffers, or to device buffers with device_outputs."""
         n = len(specs)
         arr_specs = (GenSpec * n)(*specs)
         tids = (ctypes.c_int * n)(*tree_ids)
@@ -75,7 +83,8 @@ class ServeEngine:
             arr = np.ascontiguousarray(arrivals, dtype=np.float64)
         rc = self.lib.fn2ServeReplay(self.h, n, tids, arr_specs,
                                      arr.ctypes.data_as(_lib.c_double_p) if arr is not None else None,
-                                     max_outstanding, lat.ctypes.data_as(_lib.c_double_p), ctypes.byref(wall))
+                                     max_outstanding, 1 if device_outputs else 0,
+                                     lat.ctypes.data_as(_lib.c_double_p), ctypes.byref(wall))
         if rc:
             raise RuntimeError("fn2ServeReplay failed: " + last_error())
         return lat, wall.value

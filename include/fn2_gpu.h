@@ -55,6 +55,10 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
 /* Device memory helpers (hipMalloc / hipFree / hipMemcpy, kind: 0 H2D, 1 D2H, 2 D2D). */
 FASTNOISE_API void* fn2Malloc( size_t bytes );
 FASTNOISE_API void fn2Free( void* p );
+/* Pinned (page-locked) host memory (hipHostMalloc / hipHostFree): outputs
+ * in it are written by DMA with no staging copy. */
+FASTNOISE_API void* fn2MallocHost( size_t bytes );
+FASTNOISE_API void fn2FreeHost( void* p );
 FASTNOISE_API int fn2Memcpy( void* dst, const void* src, size_t bytes, int kind );
 FASTNOISE_API int fn2DeviceSynchronize( void );
 FASTNOISE_API int fn2DeviceCount( void );
@@ -65,7 +69,8 @@ FASTNOISE_API const char* fn2GetLastError( void );
 /* Runs one of the library's device math routines over dIn[0..n) into dOut
  * (device arrays): 0 sqrt, 1 reciprocal, 2 1/sqrt (the fast correctly rounded
  * forms the cellular nodes use for positive normal arguments), 3 the IEEE
- * 1/sqrt. Lets a test compare them with IEEE results on the installed GPU. */
+ * 1/sqrt, 4 the bare hardware square root, 5 the bare hardware reciprocal.
+ * Lets a test compare them with IEEE results on the installed GPU. */
 FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n );
 
 /* Run-time specialisation of trees (hipRTC): a tree used repeatedly gets a
@@ -125,9 +130,11 @@ FASTNOISE_API void fn2ServeDestroy( void* engine );
 FASTNOISE_API int fn2ServeAddTree( void* engine, const char* encodedTree );
 
 /* Submits a request: tree id, generation spec (grid or tileable; host
- * positions are not accepted), host output buffer of the spec's point count.
- * Returns a request id (>= 0) or -1 when the request can never fit the
- * budget. The call does not block on the GPU. */
+ * positions are not accepted), output buffer of the spec's point count.
+ * The output may be pageable host memory (staged through pinned memory),
+ * pinned host memory (filled straight from the device) or device memory
+ * (a device-to-device copy). Returns a request id (>= 0) or -1 when the
+ * request can never fit the budget. The call does not block on the GPU. */
 FASTNOISE_API int64_t fn2ServeSubmit( void* engine, int treeId, const fn2GenSpec* spec, float* hostOut );
 
 /* Waits for a request; fills its output buffer and min/max. latencyMs:
@@ -149,12 +156,13 @@ FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims );
 /* Replays a request trace against the engine from the calling thread:
  * request i (tree treeIds[i], spec specs[i]) is submitted at arrivalSec[i]
  * seconds after the start (open loop; NULL = as fast as possible), with at
- * most maxOutstanding requests in flight (closed-loop bound; outputs go to a
+ * most maxOutstanding requests in flight (closed-loop bound). Outputs go to a
  * pool of pinned host buffers, which the engine fills straight from the
- * device). latencyMs[i] receives request i's submit-to-done
+ * device, or with deviceOutputs != 0 to a pool of device buffers (consumers
+ * on the GPU, no PCIe copy). latencyMs[i] receives request i's submit-to-done
  * time, wallSec the time from the start to the last completion. */
 FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,
-                                  int maxOutstanding, double* latencyMs, double* wallSec );
+                                  int maxOutstanding, int deviceOutputs, double* latencyMs, double* wallSec );
 
 #ifdef __cplusplus
 }

@@ -138,3 +138,46 @@ def test_admission_rejects_oversized_and_bad_requests(gpu):
     eng.close()
     with pytest.raises(RuntimeError, match="budget"):
         ServeEngine(hbm_budget_bytes=1 << 20, max_batch_points=1 << 22, mode=SERVE_BATCHED)
+
+
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN])
+def test_outputs_to_pageable_pinned_and_device_memory(gpu, mode):
+    # one batch holding all three destination kinds: staged through the slot's
+    # pinned buffer, DMA straight into pinned memory, device-to-device
+    nodes = [fn.Node.from_encoded(t.encode()) for t in TREES]
+    reqs = make_requests(30, seed=40 + mode)
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=mode)
+    tids = [eng.add_tree(t.encode()) for t in TREES]
+    outs = []
+    for k, (t, spec) in enumerate(reqs):
+        n = int(np.prod(list(spec.count)[:spec.dims])) if spec.kind != fn.SOURCE_TILEABLE else spec.count[0] * spec.count[1]
+        out = [None, fn.PinnedArray(n), fn.DeviceArray(n)][k % 3]
+        outs.append(out)
+    ids = [eng.submit(tids[t], spec, out) for (t, spec), out in zip(reqs, outs)]
+    for (t, spec), rid, out in zip(reqs, ids, outs):
+        got, mm, _ = eng.wait(rid)
+        if isinstance(got, fn.PinnedArray):
+            got = got.array.copy()
+        elif isinstance(got, fn.DeviceArray):
+            got = got.to_host()
+        ref, ref_mm = alone(nodes, t, spec)
+        assert_bitexact(got, ref, "output kind %s" % type(out).__name__)
+        assert mm == ref_mm
+    eng.close()
+
+
+def test_graph_replays_with_varying_request_sizes(gpu):
+    # batches whose launch shape repeats while their point counts differ: the
+    # result copies are issued per batch, outside the recorded graph
+    node = fn.Node.from_encoded(TREES[1].encode())
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=SERVE_GRAPH)
+    tid = eng.add_tree(TREES[1].encode())
+    sizes = [(548, 547), (550, 546), (200, 1), (256, 1), (100, 2), (16, 16), (511, 513), (513, 511)]
+    for rep in range(2):
+        for k, (x, y) in enumerate(sizes):
+            spec = fn.grid_spec(2, [k * 1000.0, rep * 500.0], [x, y], seed=k + rep)
+            out, mm, _ = eng.wait(eng.submit(tid, spec))
+            ref, ref_mm = node.gen_uniform_grid_2d(k * 1000.0, rep * 500.0, x, y, seed=k + rep)
+            assert_bitexact(out, ref, "graph replay size %dx%d" % (x, y))
+            assert mm == ref_mm
+    eng.close()
