@@ -146,6 +146,106 @@ namespace fn2d
         return (float)sel;
     }
 
+    // The 3^D cells around a point ordered by how many axes leave the centre
+    // cell: the centre, then faces, edges, corners. Entry k holds the offsets
+    // (0..2 per axis, 2 bits each) and the class in bits 8..10.
+    template<int D>
+    struct CellOrder
+    {
+        static constexpr int kCells = D == 2 ? 9 : D == 3 ? 27 : 81;
+        uint16_t e[kCells];
+        constexpr CellOrder() : e()
+        {
+            int n = 0;
+            for( int cls = 0; cls <= D; cls++ )
+                for( int k = 0; k < kCells; k++ )
+                {
+                    int c = 0, code = 0, t = k;
+                    for( int a = 0; a < D; a++, t /= 3 )
+                    {
+                        c += t % 3 != 1;
+                        code |= ( t % 3 ) << ( 2 * a );
+                    }
+                    if( c == cls ) e[n++] = (uint16_t)( code | ( cls << 8 ) );
+                }
+        }
+    };
+
+    // Distance functions that sum squared axis distances (Euclidean and
+    // Euclidean squared; unknown values take the same branch of calcDistance)
+    FN2D bool squaredDistance( int func ) { return func < 2 || func > 5; }
+
+    // CellularDistance for squared distances without size jitter, visiting the
+    // cells centre-first and leaving out, for the whole wave at once, every
+    // cell that cannot change the kept slots. A cell's feature point lies
+    // within |jitter| of its lattice point, so its distance is at least
+    // |lattice point - position| - |jitter|; when that exceeds the largest kept
+    // distance (with margins of 2^-10, far above the rounding of either side)
+    // the insertion network would leave every kept slot as it is. The kept
+    // slots are the k smallest distances, whatever the order cells come in, so
+    // the result is the one of the full walk.
+    template<int D, int SLOTS>
+    FN2D void cellularDistancesPruned( const CellParams& cp, const float* pos, float* dist )
+    {
+        constexpr CellOrder<D> order;
+        const int32_t prime[4] = { kPX, kPY, kPZ, kPW };
+        int32_t hx[D][3];
+        float rel[D][3];
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            const int32_t c = iadd( roundToInt( pos[a] ), -1 );
+            // the walk's hash coordinates and cell positions, stepped the same way
+            hx[a][0] = imul( c, prime[a] );
+            hx[a][1] = iadd( hx[a][0], prime[a] );
+            hx[a][2] = iadd( hx[a][1], prime[a] );
+            float fc = (float)c;
+            rel[a][0] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][1] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][2] = fc - pos[a];
+        }
+        dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
+        const float jitterBound = fabs_( cp.jitter ) * ( 1.0f + 0x1p-16f );
+        float limit = __builtin_inff();
+#pragma unroll
+        for( int k = 0; k < CellOrder<D>::kCells; k++ )
+        {
+            const int code = order.e[k], cls = code >> 8;
+            int o[D];
+#pragma unroll
+            for( int a = 0; a < D; a++ ) o[a] = ( code >> ( 2 * a ) ) & 3;
+            if( cls > 0 )
+            {
+                if( ( order.e[k - 1] >> 8 ) != cls )
+                {
+                    // squared bound beyond which a cell cannot reach the kept slots
+                    const float r = __builtin_sqrtf( dist[SLOTS - 1] * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+                    limit = r * r;
+                }
+                float pp = rel[0][o[0]] * rel[0][o[0]];
+#pragma unroll
+                for( int a = 1; a < D; a++ ) pp = fmadd( rel[a][o[a]], rel[a][o[a]], pp );
+                if( waveAll( pp > limit ) ) continue;
+            }
+            int32_t hc[D];
+            float d[D], v[D];
+#pragma unroll
+            for( int a = 0; a < D; a++ ) hc[a] = hx[a][o[a]];
+            const int32_t h = cellHash<D>( cp.seed, hc );
+            cellVector<D>( h, v );
+            const float invMag = cp.jitter * invSqrtPos( cellMagSq<D>( v ) );
+#pragma unroll
+            for( int a = 0; a < D; a++ ) d[a] = fmadd( v[a], invMag, rel[a][o[a]] );
+            const float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );
+            if constexpr( SLOTS > 3 ) dist[3] = fmaxf_( fminf_( dist[3], nd ), dist[2] );
+            if constexpr( SLOTS > 2 ) dist[2] = fmaxf_( fminf_( dist[2], nd ), dist[1] );
+            if constexpr( SLOTS > 1 ) dist[1] = fmaxf_( fminf_( dist[1], nd ), dist[0] );
+            dist[0] = fminf_( dist[0], nd );
+        }
+    }
+
     // CellularDistance (Cellular.inl:294-529): the four smallest distances.
     // Slot k of the insertion network depends only on slots below k, so a
     // caller that reads slots < SLOTS only (distance indices known when the
@@ -153,6 +253,11 @@ namespace fn2d
     template<int D, int SLOTS = 4>
     FN2D void cellularDistances( const CellParams& cp, const float* pos, float* dist )
     {
+        if( squaredDistance( cp.distFunc ) && !cp.sizeJitterOn )
+        {
+            cellularDistancesPruned<D, SLOTS>( cp, pos, dist );
+            return;
+        }
         dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
         cellWalk<D, false>( cp, pos, [&]( int32_t h, const float* v, float invMag, const float* fc ) {
             float d[D];

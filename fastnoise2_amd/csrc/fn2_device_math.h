@@ -108,6 +108,14 @@ namespace fn2d
     FN2D float sqrtPosRN( float x ) { return __builtin_sqrtf( x ); }
     FN2D float recipPosRN( float b ) { return 1.0f / b; }
 #endif
+    // true when the predicate holds on every active lane of the wave (a host
+    // build evaluates one point at a time: its own predicate)
+#if defined( __HIP_DEVICE_COMPILE__ )
+    __device__ __forceinline__ bool waveAll( bool p ) { return __all( p ); }
+#else
+    inline bool waveAll( bool p ) { return p; }
+#endif
+
     // 1 / sqrt( x ) rounded twice, as FS::InvSqrt under strict semantics
     FN2D float invSqrtPos( float x ) { return recipPosRN( sqrtPosRN( x ) ); }
     FN2D float recip( float a ) { return 1.0f / a; }

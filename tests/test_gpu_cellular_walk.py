@@ -1,0 +1,66 @@
+"""CellularDistance with squared distances visits cells centre-first and
+leaves out, per wave, cells that cannot reach the kept distance slots. These
+cases stress that walk against the oracle's full walk, bit for bit: coherent
+grids (where whole waves skip cells) and scattered positions (where they
+rarely do), every pair of distance indices, jitter values that are
+negative, zero, small, large and per-point (a node), in 2D, 3D and 4D."""
+import numpy as np
+import pytest
+
+import fastnoise2_amd as fn
+from fastnoise2_amd.tree import NodeSpec as S
+import oracle
+from conftest import assert_bitexact
+
+pytestmark = pytest.mark.gpu
+
+JITTERS = [1.0, -1.5, 0.0, 0.3, 2.5, S("Perlin", feature_scale=7.0)]
+
+
+def tree(jitter, i0, i1, func, ret="Index0Add1"):
+    return S("CellularDistance", feature_scale=9.0, distance_function=func, return_type=ret,
+             distance_index_0=i0, distance_index_1=i1, grid_jitter=jitter)
+
+
+@pytest.mark.parametrize("dims", [2, 3, 4])
+@pytest.mark.parametrize("jitter", range(len(JITTERS)), ids=["j1", "jneg", "j0", "jsmall", "jbig", "jnode"])
+def test_grid_walk_matches_oracle(gpu, dims, jitter):
+    for i0, i1, func in [(0, 1, "Euclidean"), (1, 3, "Euclidean Squared"), (2, 0, "Euclidean"), (3, 3, "Euclidean Squared")]:
+        spec = tree(JITTERS[jitter], i0, i1, func, ret="Index0Sub1")
+        node = fn.Node.from_encoded(spec.encode())
+        cnt = {2: [96, 40], 3: [64, 12, 10], 4: [20, 8, 6, 5]}[dims]
+        off = [-31.25, 7.5, 100.0, -3.0][:dims]
+        gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d, 4: node.gen_uniform_grid_4d}[dims]
+        out, mm = gen(*off, *cnt, *([0.37] * dims), seed=12345)
+        ref, ref_mm = oracle.Tree(spec.encode()).grid(off, cnt, [0.37] * dims, seed=12345)
+        assert_bitexact(out, ref, "%dD jitter %d indices %d,%d %s" % (dims, jitter, i0, i1, func))
+        assert mm == ref_mm
+
+
+@pytest.mark.parametrize("dims", [2, 3, 4])
+def test_scattered_positions_match_oracle(gpu, dims):
+    rng = np.random.default_rng(dims)
+    n = 5000
+    # mostly scattered, some clustered, some near lattice points and half-way planes
+    pos = rng.uniform(-400.0, 400.0, (dims, n)).astype(np.float32)
+    pos[:, :1000] = rng.normal(0.0, 0.3, (dims, 1000)).astype(np.float32) * 9.0
+    pos[:, 1000:1500] = (np.round(pos[:, 1000:1500] / 9.0) * 9.0).astype(np.float32)
+    pos[:, 1500:2000] = ((np.floor(pos[:, 1500:2000] / 9.0) + 0.5) * 9.0).astype(np.float32)
+    for jitter in (1.0, -0.7, 2.5):
+        spec = tree(jitter, 0, 1, "Euclidean")
+        node = fn.Node.from_encoded(spec.encode())
+        out, mm = node.gen_position_array(list(pos), seed=77)
+        ref, ref_mm = oracle.Tree(spec.encode()).positions(list(pos), seed=77)
+        assert_bitexact(out, ref, "%dD scattered jitter %g" % (dims, jitter))
+        assert mm == ref_mm
+
+
+def test_huge_and_special_coordinates(gpu):
+    spec = tree(1.0, 0, 1, "Euclidean")
+    node = fn.Node.from_encoded(spec.encode())
+    xs = np.array([1e7, -1e7, 3.0e9, 1e20, np.inf, np.nan, 0.0, -0.0, 4.5, 1e-30] * 7, dtype=np.float32)
+    ys = np.roll(xs, 3)
+    zs = np.roll(xs, 5)
+    out, _ = node.gen_position_array([xs, ys, zs], seed=5)
+    ref, _ = oracle.Tree(spec.encode()).positions([xs, ys, zs], seed=5)
+    assert_bitexact(out, ref, "huge and special coordinates")

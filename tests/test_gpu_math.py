@@ -48,30 +48,3 @@ def test_fast_math_is_correctly_rounded(gpu, op, lo, hi):
         bad += n
     assert bad == 0, "%d mismatches, first (x, got, ieee) = %r" % (bad, first)
 
-
-def cell_magnitudes(dims):
-    """Every squared length a cellular jitter vector can have: its components
-    are (odd integer) / 2 with |odd| <= 2^bits - 1 (11 bits in 2D, 10 in 3D,
-    8 in 4D), so the length is N / 4 for a sum N of `dims` odd squares, which
-    is N = dims (mod 8); all such N up to the maximum (a superset)."""
-    top = {2: 2047, 3: 1023, 4: 255}[dims]
-    n = np.arange(dims, dims * top * top + 1, 8, dtype=np.float64)
-    return (n / 4).astype(np.float32)
-
-
-@pytest.mark.parametrize("dims", [2, 3, 4])
-def test_bare_sqrt_and_reciprocal_exact_on_cell_magnitudes(gpu, dims):
-    # the cellular nodes' 1 / sqrt(|v|^2) on the exact values it can take:
-    # whether the bare hardware square root and reciprocal round correctly there
-    x = cell_magnitudes(dims)
-    assert (x.astype(np.float64) * 4 == np.arange(dims, dims * {2: 2047, 3: 1023, 4: 255}[dims] ** 2 + 1, 8)).all()
-    din = fn.DeviceArray(x.size)
-    dout = fn.DeviceArray(x.size)
-    s = run(4, x, din, dout)
-    s_ref = np.sqrt(x)
-    bad_s = int((s.view(np.uint32) != s_ref.view(np.uint32)).sum())
-    r = run(5, s_ref, din, dout)
-    r_ref = np.float32(1.0) / s_ref
-    bad_r = int((r.view(np.uint32) != r_ref.view(np.uint32)).sum())
-    print("dims %d: %d values, sqrt mismatches %d, reciprocal mismatches %d" % (dims, x.size, bad_s, bad_r))
-    assert bad_s == 0 and bad_r == 0, (bad_s, bad_r)
