@@ -26,9 +26,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Gpoints/s per graph (GPU vs host SIMD); p50/p99 chunk latency batched vs plain"
-FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # 256 CUs x 64 lanes per clock x 2.4 GHz, one op per lane
+FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak (an FMA counted as 2)
+# vector ALU issue peak behind it: 157.3 / 2 = 78.6 T lane operations per second
+# (256 CUs x 128 lanes per clock x 2.4 GHz), one operation per lane per instruction
+VALU_PEAK_TOPS = FP32_PEAK_TFLOPS / 2
 HBM_PEAK_GBS = 8000.0
+COPY_CAL_MIB = 256                # device-to-device copy that scales the HBM counters
 
 
 def make_spec(fn, W, wl, keep):
@@ -97,6 +100,69 @@ class _RowView:
 
     def __getitem__(self, sl):
         return [a[sl] for a in self.arrays]
+
+
+def pmc_child(fn, W, wl, name):
+    """Counter pass body: three 256 MiB device-to-device copies (the scale
+    reference), then the workload's kernel three times; prints its name."""
+    nbytes = COPY_CAL_MIB << 20
+    src, dst = fn.DeviceArray(nbytes // 4), fn.DeviceArray(nbytes // 4)
+    for _ in range(3):
+        if fn.lib().fn2Memcpy(dst.ptr, src.ptr, nbytes, 2):
+            raise SystemExit("copy failed: " + fn.last_error())
+    node = fn.Node.from_encoded(wl["tree"]().encode())
+    keep = []
+    spec = make_spec(fn, W, wl, keep)
+    out = fn.DeviceArray(int(np.prod(wl["count"])))
+    fn.set_jit_mode(fn.JIT_SYNC)
+    node.bench_device(spec, out, warmup=0, iters=1)
+    fn.set_jit_mode(fn.JIT_ASYNC)
+    node.bench_device(spec, out, warmup=0, iters=3)
+    dims = wl["dims"] if wl["kind"] != "tileable" else 4
+    print(json.dumps({"kernel": "fn2_jit_single" if node.jit_state(dims) == 1 else "fn2_generate"}), flush=True)
+
+
+def measure_traffic(workload):
+    """HBM bytes per launch of the workload's kernel: one rocprofv3 --pmc pass
+    for FETCH_SIZE, one for WRITE_SIZE (KiB per dispatch), each scaled by what
+    the same counter reads for the 256 MiB copy (which moves 256 MiB each way).
+    None when rocprofv3 is missing or a pass fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="fn2_pmc_")
+        cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", counter, "--kernel-trace", "-f", "csv",
+               "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--workload", workload,
+               "--pmc-child"]
+        p = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, FN2_JIT="async"))
+        kernel = None
+        for line in p.stdout.splitlines():
+            if line.startswith("{"):
+                kernel = json.loads(line).get("kernel")
+        rows = []
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    with open(os.path.join(root, f)) as fh:
+                        rows += list(csv.DictReader(fh))
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [float(r["Counter_Value"]) for r in rows
+                if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer") and int(r["Grid_Size"]) >= 1 << 16]
+        gen = [float(r["Counter_Value"]) for r in rows if kernel and kernel in r["Kernel_Name"]]
+        if not copy or len(gen) < 2:
+            progress("%s pass gave no usable rows (exit %d, kernel %r, %d rows); its output ends with: %s"
+                     % (counter, p.returncode, kernel, len(rows), (p.stdout + p.stderr)[-1500:]))
+            return None
+        scale = (COPY_CAL_MIB << 10) / (sum(copy) / len(copy))
+        kib = sum(gen[1:]) / len(gen[1:])   # the first dispatch includes compile-time warm-up
+        res[counter] = {"kib_per_dispatch": round(kib, 1), "copy_scale": round(scale, 3)}
+    total = sum(v["kib_per_dispatch"] * v["copy_scale"] * 1024 for v in res.values())
+    return {"bytes": int(total), "kernel": kernel, **res}
 
 
 def progress(msg):
@@ -198,16 +264,28 @@ def main():
                     help="run-time specialised kernel of the tree (compiled before timing) or the interpreter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the reference kernel for HBM counter scaling)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes for roofline.traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a single MI355X; "
               "multi-GPU sharding is described in DESIGN.md but not run (got --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import fastnoise2_amd as fn
     import fastnoise2_amd.workloads as W
+    wls = W.bench_workloads()
+    # counter passes first: they run in child processes, started before this
+    # process touches the GPU
+    traffic = None
+    if not args.no_traffic and not args.pmc_child and not args.copy_calibration:
+        traffic = measure_traffic(args.workload)
+        progress("hbm counter passes done")
+    import fastnoise2_amd as fn
     if fn.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.pmc_child:
+        pmc_child(fn, W, wls[args.workload], args.workload)
+        return
     if args.copy_calibration:
         nbytes = args.copy_calibration << 20
         src, dst = fn.DeviceArray(nbytes // 4), fn.DeviceArray(nbytes // 4)
@@ -216,7 +294,6 @@ def main():
                 raise SystemExit("copy failed: " + fn.last_error())
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return
-    wls = W.bench_workloads()
     wl = wls[args.workload]
     spec_tree = wl["tree"]()
     enc = spec_tree.encode()
@@ -248,20 +325,25 @@ def main():
     kernel_s = ms_event * 1e-3
     tflops = flop_pt * n / kernel_s / 1e12
     bytes_pt = 4 + (4 * wl["dims"] if wl["kind"] == "positions" else 0)
+    valu_tops = valu_pt * n / kernel_s / 1e12
     roofline = {
+        # vector-ALU bound: integer hashing and per-lane FP32, no contraction
+        # for MFMA and 4 bytes of HBM traffic per point
         "bound": "valu",
-        "achieved": round(tflops, 3),
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
-        "traffic": None,
+        "achieved": round(valu_tops, 3),
+        "peak": round(VALU_PEAK_TOPS, 2),
+        "unit": "T VALU lane-ops/s",
+        "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
+        "traffic": traffic["bytes"] if traffic else None,
+        "traffic_detail": traffic,
+        "traffic_algorithmic_bytes": bytes_pt * n,
         "kernel": kernel_name,
         "interpreter_kernel_ms": round(interp_ms, 4),
         "kernel_ms": round(ms_event, 4),
-        "flop_per_point": flop_pt,
         "valu_ops_per_point": valu_pt,
-        "valu_achieved_Tops": round(valu_pt * n / kernel_s / 1e12, 3),
-        "valu_peak_Tops": round(VALU_PEAK_TOPS, 2),
+        "flop_per_point": flop_pt,
+        "fp32_TFLOPs": round(tflops, 3),
+        "fp32_peak_TFLOPs": FP32_PEAK_TFLOPS,
         "hbm_algorithmic_GBs": round(bytes_pt * n / kernel_s / 1e9, 1),
         "hbm_peak_GBs": HBM_PEAK_GBS,
     }

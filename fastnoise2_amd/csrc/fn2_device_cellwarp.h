@@ -206,7 +206,7 @@ namespace fn2d
             fc = fc + 1.0f;
             rel[a][2] = fc - pos[a];
         }
-        dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
+        uint32_t key[4] = { 0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u };   // +inf
         const float jitterBound = fabs_( cp.jitter ) * ( 1.0f + 0x1p-16f );
         float limit = __builtin_inff();
 #pragma unroll
@@ -221,7 +221,7 @@ namespace fn2d
                 if( ( order.e[k - 1] >> 8 ) != cls )
                 {
                     // squared bound beyond which a cell cannot reach the kept slots
-                    const float r = __builtin_sqrtf( dist[SLOTS - 1] * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+                    const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
                     limit = r * r;
                 }
                 float pp = rel[0][o[0]] * rel[0][o[0]];
@@ -239,24 +239,37 @@ namespace fn2d
 #pragma unroll
             for( int a = 0; a < D; a++ ) d[a] = fmadd( v[a], invMag, rel[a][o[a]] );
             const float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );
-            if constexpr( SLOTS > 3 ) dist[3] = fmaxf_( fminf_( dist[3], nd ), dist[2] );
-            if constexpr( SLOTS > 2 ) dist[2] = fmaxf_( fminf_( dist[2], nd ), dist[1] );
-            if constexpr( SLOTS > 1 ) dist[1] = fmaxf_( fminf_( dist[1], nd ), dist[0] );
-            dist[0] = fminf_( dist[0], nd );
+            // Squared distances are +0 or more (or NaN), and on such floats
+            // the unsigned order of the bit patterns is the float order with NaN
+            // above +inf: unsigned min / max give the results of the float
+            // min / max steps above (a NaN distance never displaces a slot, and
+            // no slot becomes NaN), without canonicalising their operands.
+            const uint32_t nb = (uint32_t)fbits( nd );
+            if constexpr( SLOTS > 3 ) key[3] = umax( umin( key[3], nb ), key[2] );
+            if constexpr( SLOTS > 2 ) key[2] = umax( umin( key[2], nb ), key[1] );
+            if constexpr( SLOTS > 1 ) key[1] = umax( umin( key[1], nb ), key[0] );
+            key[0] = umin( key[0], nb );
         }
+#pragma unroll
+        for( int k = 0; k < 4; k++ ) dist[k] = ibits( (int32_t)key[k] );
     }
 
     // CellularDistance (Cellular.inl:294-529): the four smallest distances.
     // Slot k of the insertion network depends only on slots below k, so a
     // caller that reads slots < SLOTS only (distance indices known when the
     // kernel is compiled) may keep just those: they hold the same values.
-    template<int D, int SLOTS = 4>
+    // PRUNED: take the pruned walk where it applies (specialised kernels; the
+    // interpreter keeps the one walk, whose registers it can afford)
+    template<int D, int SLOTS = 4, bool PRUNED = false>
     FN2D void cellularDistances( const CellParams& cp, const float* pos, float* dist )
     {
-        if( squaredDistance( cp.distFunc ) && !cp.sizeJitterOn )
+        if constexpr( PRUNED )
         {
-            cellularDistancesPruned<D, SLOTS>( cp, pos, dist );
-            return;
+            if( squaredDistance( cp.distFunc ) && !cp.sizeJitterOn )
+            {
+                cellularDistancesPruned<D, SLOTS>( cp, pos, dist );
+                return;
+            }
         }
         dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
         cellWalk<D, false>( cp, pos, [&]( int32_t h, const float* v, float invMag, const float* fc ) {

@@ -108,6 +108,9 @@ namespace fn2d
     FN2D float sqrtPosRN( float x ) { return __builtin_sqrtf( x ); }
     FN2D float recipPosRN( float b ) { return 1.0f / b; }
 #endif
+    FN2D uint32_t umin( uint32_t a, uint32_t b ) { return a < b ? a : b; }
+    FN2D uint32_t umax( uint32_t a, uint32_t b ) { return a > b ? a : b; }
+
     // true when the predicate holds on every active lane of the wave (a host
     // build evaluates one point at a time: its own predicate)
 #if defined( __HIP_DEVICE_COMPILE__ )

@@ -13,6 +13,12 @@
 #include "fn2_device_cellwarp.h"
 #include "fn2_kernel_args.h"
 
+// The interpreter takes the full cell walk; a host build may select the
+// pruned one to check it point by point
+#ifndef FN2_PRUNED_CELL_WALK
+#define FN2_PRUNED_CELL_WALK false
+#endif
+
 namespace fn2i
 {
     using namespace fn2d;
@@ -148,7 +154,7 @@ namespace fn2i
             if( in.op == fn2::OP_CELL_DISTANCE )
             {
                 float dist[4];
-                cellularDistances<D, SLOTS>( cp, pos, dist );
+                cellularDistances<D, SLOTS, ( OP >= 0 ) || FN2_PRUNED_CELL_WALK>( cp, pos, dist );
                 R[in.r[0]] = scaleOutput( cellularReturn( dist, in.p[0], in.p[1], in.p[2], in.p[3], 0.0f ), K[1], K[2], K[4] );
                 return;
             }
