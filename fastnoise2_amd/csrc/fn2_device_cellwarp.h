@@ -190,7 +190,7 @@ namespace fn2d
         constexpr CellOrder<D> order;
         const int32_t prime[4] = { kPX, kPY, kPZ, kPW };
         int32_t hx[D][3];
-        float rel[D][3], rel2[D][3];
+        float rel[D][3];
 #pragma unroll
         for( int a = 0; a < D; a++ )
         {
@@ -205,30 +205,10 @@ namespace fn2d
             rel[a][1] = fc - pos[a];
             fc = fc + 1.0f;
             rel[a][2] = fc - pos[a];
-#pragma unroll
-            for( int i = 0; i < 3; i++ ) rel2[a][i] = rel[a][i] * rel[a][i];
         }
         uint32_t key[4] = { 0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u };   // +inf
         const float jitterBound = fabs_( cp.jitter ) * ( 1.0f + 0x1p-16f );
-        // the squared distance to a cell's lattice point is the centre cell's
-        // plus, per axis that leaves the centre, that axis' step cost; the
-        // nearest cell of class c adds the c cheapest steps
-        float centre = rel2[0][1], step[D];
-#pragma unroll
-        for( int a = 1; a < D; a++ ) centre += rel2[a][1];
-#pragma unroll
-        for( int a = 0; a < D; a++ ) step[a] = fminf_( rel2[a][0], rel2[a][2] ) - rel2[a][1];
-#pragma unroll
-        for( int i = 0; i < D; i++ )   // ascending
-#pragma unroll
-            for( int j = 0; j + 1 < D - i; j++ )
-            {
-                const float lo = fminf_( step[j], step[j + 1] ), hi = fmaxf_( step[j], step[j + 1] );
-                step[j] = lo;
-                step[j + 1] = hi;
-            }
         float limit = __builtin_inff();
-        bool skipClass = false;
 #pragma unroll
         for( int k = 0; k < CellOrder<D>::kCells; k++ )
         {
@@ -243,16 +223,10 @@ namespace fn2d
                     // squared bound beyond which a cell cannot reach the kept slots
                     const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
                     limit = r * r;
-                    // the whole class at once when even its nearest cell is out of reach
-                    float nearest = centre;
-#pragma unroll
-                    for( int a = 0; a < cls; a++ ) nearest += step[a];
-                    skipClass = cls > 1 && waveAll( nearest > limit );
                 }
-                if( skipClass ) continue;
-                float pp = rel2[0][o[0]];
+                float pp = rel[0][o[0]] * rel[0][o[0]];
 #pragma unroll
-                for( int a = 1; a < D; a++ ) pp += rel2[a][o[a]];
+                for( int a = 1; a < D; a++ ) pp = fmadd( rel[a][o[a]], rel[a][o[a]], pp );
                 if( waveAll( pp > limit ) ) continue;
             }
             int32_t hc[D];
