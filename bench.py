@@ -251,6 +251,48 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
     return out
 
 
+def world_bench(W, chunk_size=64, load_range=400.0, step=16.0, target_chunks=3000, max_seconds=10.0):
+    """Chunked-world fly-through: a camera moving along x by `step` units per
+    frame through the terrain tree; each frame updates the world (requests
+    the missing chunks in range, nearest first) and collects finished ones
+    (range culling and face counts on the GPU). Chunks per second, their
+    points per second and request-to-consumer latency, batched vs plain."""
+    import fastnoise2_amd as fn
+    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED
+    from fastnoise2_amd.world import ChunkWorld, CHUNK_SURFACE
+    enc = W.world_tree().encode()
+    out = {"tree": enc, "chunk": "%d^3 (+2 margin)" % chunk_size, "load_range": load_range, "camera_step": step}
+    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED)):
+        eng = ServeEngine(16 << 30, 1 << 23, mode)
+        tid = eng.add_tree(enc)
+        world = ChunkWorld(eng, tid, chunk_size=chunk_size, iso_surface=0.0, load_range=load_range, seed=1337,
+                           max_in_flight=128)
+        world.update((0.0, 0.0, 0.0))
+        world.poll(wait_all=True)        # compile, tune and warm up
+        fn.jit_wait()
+        world.new_version()
+        lat, surface, frames = [], 0, 0
+        cam = [0.0, 0.0, 0.0]
+        t0 = time.perf_counter()
+        while len(lat) < target_chunks and time.perf_counter() - t0 < max_seconds:
+            world.update(cam)
+            for r in world.poll():
+                lat.append(r["latency_ms"])
+                surface += r["state"] == CHUNK_SURFACE
+            cam[0] += step
+            frames += 1
+        wall = time.perf_counter() - t0
+        g = chunk_size + 2
+        out[name] = {"chunks_s": round(len(lat) / wall, 1), "Gpoints_s": round(len(lat) * g ** 3 / wall / 1e9, 3),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "surface_fraction": round(surface / max(1, len(lat)), 3), "frames": frames, "chunks": len(lat)}
+        world.close()
+        eng.close()
+        progress("world %s done" % name)
+    out["batched_vs_plain_chunks"] = round(out["batched"]["chunks_s"] / out["plain"]["chunks_s"], 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -381,6 +423,7 @@ def main():
     if not args.no_serving and args.workload == "fbm_simplex_2d_grid_4096x4096":
         rec["serving"] = serving_bench(W)
         rec["serving_mixed"] = serving_mixed_bench(W)
+        rec["world"] = world_bench(W)
     print(json.dumps(rec), flush=True)
 
 

@@ -115,6 +115,14 @@ GPU_EXT = {
     "fn2ServeDrain": (i, [vp]),
     "fn2ServeStats": (i, [vp, c_int64_p, c_int64_p, c_int64_p]),
     "fn2ServeTreeTuning": (i, [vp, i, i]),
+    "fn2ServePoll": (i, [vp, ctypes.c_int64, c_float_p, c_double_p]),
+    "fn2WorldChunksInRange": (i, [c_float_p, i, ctypes.c_float, ctypes.POINTER(ctypes.c_int32), i]),
+    "fn2WorldCreate": (vp, [vp, i, i, ctypes.c_float, ctypes.c_float, i, i]),
+    "fn2WorldDestroy": (None, [vp]),
+    "fn2WorldUpdate": (i, [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "fn2WorldPoll": (i, [vp, vp, i, i]),
+    "fn2WorldNewVersion": (ctypes.c_uint32, [vp]),
+    "fn2WorldStats": (i, [vp, c_int64_p, c_int64_p, c_int64_p, c_int_p, c_int_p]),
     "fn2ServeReplay": (i, [vp, i, c_int_p, ctypes.POINTER(GenSpec), c_double_p, i, i, c_double_p, c_double_p]),
 }
 

@@ -56,6 +56,46 @@ __global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
     }
 }
 
+// Consumer of a chunk's (s + 2)^3 density block (x fastest): over the s^3
+// interior voxels, counts the solid ones (value <= iso) and their faces
+// toward a non-solid neighbour (value > iso), the quads of a bloxel mesh
+// (MeshNoisePreview.cpp:457-520 of the reference). counts = { quads, solid }.
+__global__ void __launch_bounds__( 256 ) fn2_bloxel_faces( const float* __restrict__ d, int g, float iso, uint32_t* counts )
+{
+    const uint32_t s = (uint32_t)g - 2, total = s * s * s;
+    const uint32_t gg = (uint32_t)g * (uint32_t)g;
+    uint32_t quads = 0, solid = 0;
+    for( uint32_t i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256 )
+    {
+        const uint32_t x = i % s, y = ( i / s ) % s, z = i / ( s * s );
+        const uint32_t idx = ( z + 1 ) * gg + ( y + 1 ) * (uint32_t)g + ( x + 1 );
+        if( d[idx] <= iso )
+        {
+            solid++;
+            quads += ( d[idx + 1] > iso ) + ( d[idx - 1] > iso ) + ( d[idx + g] > iso ) + ( d[idx - g] > iso ) +
+                     ( d[idx + gg] > iso ) + ( d[idx - gg] > iso );
+        }
+    }
+    __shared__ uint32_t sq[4], ss[4];
+#pragma unroll
+    for( int off = 32; off > 0; off >>= 1 )
+    {
+        quads += __shfl_xor( quads, off );
+        solid += __shfl_xor( solid, off );
+    }
+    if( ( threadIdx.x & 63 ) == 0 )
+    {
+        sq[threadIdx.x >> 6] = quads;
+        ss[threadIdx.x >> 6] = solid;
+    }
+    __syncthreads();
+    if( threadIdx.x == 0 )
+    {
+        atomicAdd( &counts[0], sq[0] + sq[1] + sq[2] + sq[3] );
+        atomicAdd( &counts[1], ss[0] + ss[1] + ss[2] + ss[3] );
+    }
+}
+
 __global__ void fn2_minmax_init( int32_t* keys, int n )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -148,6 +188,17 @@ int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStr
 {
     if( n <= 0 ) return 0;
     hipLaunchKernelGGL( fn2_selftest_math, dim3( ( n + 255 ) / 256 ), dim3( 256 ), 0, stream, op, in, out, n );
+    return (int)hipGetLastError();
+}
+
+int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t* counts, hipStream_t stream )
+{
+    // interior voxels (gen - 2)^3 stay below 2^31 and every neighbour index
+    // inside the gen^3 block for 3 <= gen <= 1290
+    if( !density || !counts || gen < 3 || gen > 1290 ) return (int)hipErrorInvalidValue;
+    const int64_t s = gen - 2, total = s * s * s;
+    const unsigned blocks = std::max( 1u, GridFor( total ) );
+    hipLaunchKernelGGL( fn2_bloxel_faces, dim3( blocks ), dim3( 256 ), 0, stream, density, gen, iso, counts );
     return (int)hipGetLastError();
 }
 

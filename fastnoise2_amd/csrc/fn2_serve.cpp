@@ -446,6 +446,22 @@ namespace
             return status;
         }
 
+        // Non-blocking Wait: 1 done (the request is retired), 0 still pending, -1 failed
+        int Poll( int64_t id, float* mm, double* latencyMs )
+        {
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                auto it = requests.find( id );
+                if( it == requests.end() )
+                {
+                    fn2::SetError( "unknown request id" );
+                    return -1;
+                }
+                if( !it->second->done ) return 0;
+            }
+            return Wait( id, mm, latencyMs ) ? -1 : 1;
+        }
+
         int Drain()
         {
             std::unique_lock<std::mutex> lk( mu );
@@ -855,6 +871,13 @@ FASTNOISE_API int fn2ServeWait( void* engine, int64_t requestId, float* outputMi
     fn2::ClearError();
     if( !engine ) { fn2::SetError( "null engine" ); return 1; }
     return static_cast<Engine*>( engine )->Wait( requestId, outputMinMax, latencyMs );
+}
+
+FASTNOISE_API int fn2ServePoll( void* engine, int64_t requestId, float* outputMinMax, double* latencyMs )
+{
+    fn2::ClearError();
+    if( !engine ) { fn2::SetError( "null engine" ); return -1; }
+    return static_cast<Engine*>( engine )->Poll( requestId, outputMinMax, latencyMs );
 }
 
 FASTNOISE_API int fn2ServeDrain( void* engine )

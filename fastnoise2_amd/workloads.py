@@ -48,6 +48,14 @@ def serving_tree():
     return S("Add", lhs=remap, rhs=cell)
 
 
+def world_tree():
+    """Terrain density for the chunked-world client: a height gradient
+    (value 0.01 per unit of y) plus FractalFBm(Perlin, 4 octaves, frequency
+    0.01); the iso surface 0 lies near y = 0."""
+    return S("Add", lhs=S("Gradient", multiplier_y=0.01),
+             rhs=S("FractalFBm", source=S("Perlin", feature_scale=100.0), octaves=4))
+
+
 def splitmix64_positions(n, seed=7, lo=-1000.0, hi=1000.0):
     """SplitMix64 stream seeded `seed`; draws 3i, 3i+1, 3i+2 give point i's
     x, y, z = lo + (hi - lo) * ((r >> 40) * 2^-24), computed in float64 and

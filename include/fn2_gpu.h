@@ -142,6 +142,10 @@ FASTNOISE_API int fn2ServeWait( void* engine, int64_t requestId, float* outputMi
 
 /* Waits for everything submitted so far. */
 FASTNOISE_API int fn2ServeDrain( void* engine );
+/* Non-blocking form of fn2ServeWait: 1 when the request is done (it is then
+ * retired and its range and latency written), 0 while it is pending, -1 on
+ * failure or an unknown id. */
+FASTNOISE_API int fn2ServePoll( void* engine, int64_t requestId, float* outputMinMax, double* latencyMs );
 
 /* Engine counters: batches launched, requests completed, device bytes held. */
 FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* completed, int64_t* peakBytes );
@@ -162,6 +166,50 @@ FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims );
  * time, wallSec the time from the start to the last completion. */
 FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,
                                   int maxOutstanding, int deviceOutputs, double* latencyMs, double* wallSec );
+
+/* ------------------------------------------------------------------------
+ * Chunked-world client: streams a world of (size + 2)^3 density blocks
+ * around a moving camera through a serving engine, nearest chunks first, and
+ * consumes each block on the GPU (range culling against an iso surface, then
+ * a count of the exposed voxel faces a bloxel mesher would emit). Only the
+ * per-chunk summary reaches the host. Destroy the world before its engine.
+ * ------------------------------------------------------------------------ */
+enum fn2ChunkState
+{
+    FN2_CHUNK_AIR = 0,      /* every value above the iso surface */
+    FN2_CHUNK_SOLID = 1,    /* every value below it */
+    FN2_CHUNK_SURFACE = 2,  /* straddles it: faces counted */
+};
+
+typedef struct fn2ChunkResult
+{
+    int32_t x, y, z;        /* chunk origin (a multiple of the chunk size) */
+    int32_t state;          /* fn2ChunkState */
+    float min, max;         /* value range of the whole block, margins included */
+    uint32_t quads;         /* solid voxels' faces toward non-solid neighbours */
+    uint32_t solidVoxels;   /* voxels at or below the iso surface (interior) */
+    double latencyMs;       /* request to consumer done */
+    uint32_t version;       /* generation the chunk belongs to */
+} fn2ChunkResult;
+
+/* Chunk origins within loadRange of a camera, nearest first (three int32 per
+ * chunk, at most maxChunks written); returns the full count or -1. */
+FASTNOISE_API int fn2WorldChunksInRange( const float* camera, int chunkSize, float loadRange, int32_t* chunkOrigins, int maxChunks );
+/* A world over tree treeId of the engine, with at most maxInFlight chunks
+ * requested at once (each holds a device density block). */
+FASTNOISE_API void* fn2WorldCreate( void* engine, int treeId, int chunkSize, float isoSurface, float loadRange, int seed, int maxInFlight );
+FASTNOISE_API void fn2WorldDestroy( void* world );
+/* Unloads chunks beyond 1.1 x the load range and requests the missing ones in
+ * range, nearest first, while blocks are free; returns the number requested. */
+FASTNOISE_API int fn2WorldUpdate( void* world, float cameraX, float cameraY, float cameraZ );
+/* Writes up to maxResults finished chunks (oldest first) and returns their
+ * number; with waitAll it waits for every chunk in flight. Chunks of a
+ * superseded version are dropped, never returned. */
+FASTNOISE_API int fn2WorldPoll( void* world, fn2ChunkResult* results, int maxResults, int waitAll );
+/* Starts a new generation (a teleport or a changed tree): chunks in flight
+ * are dropped when they finish and every chunk is requested again. */
+FASTNOISE_API uint32_t fn2WorldNewVersion( void* world );
+FASTNOISE_API int fn2WorldStats( void* world, int64_t* requested, int64_t* completed, int64_t* dropped, int* loaded, int* inFlight );
 
 #ifdef __cplusplus
 }
