@@ -251,17 +251,20 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
     return out
 
 
-def world_bench(W, chunk_size=64, load_range=400.0, step=16.0, target_chunks=3000, max_seconds=10.0):
-    """Chunked-world fly-through: a camera moving along x by `step` units per
-    frame through the terrain tree; each frame updates the world (requests
-    the missing chunks in range, nearest first) and collects finished ones
-    (range culling and face counts on the GPU). Chunks per second, their
-    points per second and request-to-consumer latency, batched vs plain."""
+def world_bench(W, chunk_size=64, load_range=400.0, step=64.0, frames=16):
+    """Chunked-world fly-through: a camera stepping along x by one chunk per
+    frame through the terrain tree; each frame streams every missing chunk in
+    range (nearest first, at most 128 in flight) before the camera moves on,
+    so both modes deliver the same chunks (the first frame's sphere, then a
+    slab per frame). Range culling and face counts run on the GPU. Chunks per
+    second, their points per second and request-to-consumer latency, batched
+    vs plain."""
     import fastnoise2_amd as fn
     from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED
     from fastnoise2_amd.world import ChunkWorld, CHUNK_SURFACE
     enc = W.world_tree().encode()
-    out = {"tree": enc, "chunk": "%d^3 (+2 margin)" % chunk_size, "load_range": load_range, "camera_step": step}
+    out = {"tree": enc, "chunk": "%d^3 (+2 margin)" % chunk_size, "load_range": load_range, "frames": frames,
+           "camera_step": step}
     for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED)):
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tid = eng.add_tree(enc)
@@ -271,21 +274,23 @@ def world_bench(W, chunk_size=64, load_range=400.0, step=16.0, target_chunks=300
         world.poll(wait_all=True)        # compile, tune and warm up
         fn.jit_wait()
         world.new_version()
-        lat, surface, frames = [], 0, 0
-        cam = [0.0, 0.0, 0.0]
+        lat, surface = [], 0
         t0 = time.perf_counter()
-        while len(lat) < target_chunks and time.perf_counter() - t0 < max_seconds:
-            world.update(cam)
-            for r in world.poll():
-                lat.append(r["latency_ms"])
-                surface += r["state"] == CHUNK_SURFACE
-            cam[0] += step
-            frames += 1
+        for f in range(frames):
+            cam = (f * step, 0.0, 0.0)
+            while True:
+                n = world.update(cam)
+                for r in world.poll(wait_all=True):
+                    lat.append(r["latency_ms"])
+                    surface += r["state"] == CHUNK_SURFACE
+                if n == 0:
+                    break
         wall = time.perf_counter() - t0
         g = chunk_size + 2
-        out[name] = {"chunks_s": round(len(lat) / wall, 1), "Gpoints_s": round(len(lat) * g ** 3 / wall / 1e9, 3),
+        out[name] = {"chunks": len(lat), "chunks_s": round(len(lat) / wall, 1),
+                     "Gpoints_s": round(len(lat) * g ** 3 / wall / 1e9, 3),
                      "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                     "surface_fraction": round(surface / max(1, len(lat)), 3), "frames": frames, "chunks": len(lat)}
+                     "surface_fraction": round(surface / max(1, len(lat)), 3), "wall_s": round(wall, 3)}
         world.close()
         eng.close()
         progress("world %s done" % name)
