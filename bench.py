@@ -251,7 +251,7 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
     return out
 
 
-def world_bench(W, chunk_size=64, load_range=400.0, step=64.0, frames=16):
+def world_bench(W, chunk_size=64, load_range=400.0, step=64.0, frames=48):
     """Chunked-world fly-through: a camera stepping along x by one chunk per
     frame through the terrain tree; each frame streams every missing chunk in
     range (nearest first, at most 128 in flight) before the camera moves on,
