@@ -823,7 +823,7 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
 FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n )
 {
     ClearError();
-    if( op < 0 || op > 3 || n < 0 ) { SetError( "bad self-test arguments" ); return 1; }
+    if( op < 0 || op > 5 || n < 0 ) { SetError( "bad self-test arguments" ); return 1; }
     if( !EnsureDevice() ) return 1;
     if( !HipOk( (hipError_t)fn2_launch_selftest_math( op, dIn, dOut, n, nullptr ), "self-test launch" ) ) return 1;
     return HipOk( hipDeviceSynchronize(), "self-test" ) ? 0 : 1;

@@ -102,7 +102,7 @@ namespace fn2d
                         int32_t h = cellHash<D>( cp.seed, hc );
                         float v[D];
                         cellVector<D>( h, v );
-                        float invMag = cp.jitter * invSqrtPos( cellMagSq<D>( v ) );
+                        float invMag = cp.jitter * invSqrtCell<D>( cellMagSq<D>( v ) );
                         cell( h, v, invMag, fc );
                         if constexpr( D > 3 ) { fc[3] = fc[3] + 1.0f; hc[3] = iadd( hc[3], prime[3] ); }
                     }
@@ -235,7 +235,7 @@ namespace fn2d
             for( int a = 0; a < D; a++ ) hc[a] = hx[a][o[a]];
             const int32_t h = cellHash<D>( cp.seed, hc );
             cellVector<D>( h, v );
-            const float invMag = cp.jitter * invSqrtPos( cellMagSq<D>( v ) );
+            const float invMag = cp.jitter * invSqrtCell<D>( cellMagSq<D>( v ) );
 #pragma unroll
             for( int a = 0; a < D; a++ ) d[a] = fmadd( v[a], invMag, rel[a][o[a]] );
             const float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );

@@ -104,9 +104,22 @@ namespace fn2d
         const float e = __builtin_fmaf( -b, y, 1.0f );   // exact residual
         return __builtin_fmaf( e, y, y );
     }
+
+    // sqrtPosRN for arguments on which the hardware estimate is never above
+    // the correctly rounded root, so only the step up needs testing: the
+    // squared lengths of 3D and 4D cellular jitter vectors (N / 4 for a sum N
+    // of three or four odd squares below 2^22), checked exhaustively on the
+    // device by the math tests
+    FN2D float sqrtPosRNFromBelow( float x )
+    {
+        const float s = __builtin_amdgcn_sqrtf( x );
+        const float up = ibits( fbits( s ) + 1 );
+        return __builtin_fmaf( -up, s, x ) > 0.0f ? up : s;
+    }
 #else
     FN2D float sqrtPosRN( float x ) { return __builtin_sqrtf( x ); }
     FN2D float recipPosRN( float b ) { return 1.0f / b; }
+    FN2D float sqrtPosRNFromBelow( float x ) { return __builtin_sqrtf( x ); }
 #endif
     FN2D uint32_t umin( uint32_t a, uint32_t b ) { return a < b ? a : b; }
     FN2D uint32_t umax( uint32_t a, uint32_t b ) { return a > b ? a : b; }
@@ -121,6 +134,8 @@ namespace fn2d
 
     // 1 / sqrt( x ) rounded twice, as FS::InvSqrt under strict semantics
     FN2D float invSqrtPos( float x ) { return recipPosRN( sqrtPosRN( x ) ); }
+    // the same for the squared length of a D-dimensional cellular jitter vector
+    template<int D> FN2D float invSqrtCell( float x ) { return recipPosRN( D == 2 ? sqrtPosRN( x ) : sqrtPosRNFromBelow( x ) ); }
     FN2D float recip( float a ) { return 1.0f / a; }
 
     // ---- hashing (Utils.inl:221-250)

@@ -52,7 +52,8 @@ __global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
     if( i < n )
     {
         float x = in[i];
-        out[i] = op == 0 ? sqrtPosRN( x ) : op == 1 ? recipPosRN( x ) : op == 2 ? invSqrtPos( x ) : invSqrt( x );
+        out[i] = op == 0 ? sqrtPosRN( x ) : op == 1 ? recipPosRN( x ) : op == 2 ? invSqrtPos( x ) : op == 3 ? invSqrt( x )
+               : op == 4 ? __builtin_amdgcn_sqrtf( x ) : sqrtPosRNFromBelow( x );
     }
 }
 

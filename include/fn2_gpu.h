@@ -69,7 +69,9 @@ FASTNOISE_API const char* fn2GetLastError( void );
 /* Runs one of the library's device math routines over dIn[0..n) into dOut
  * (device arrays): 0 sqrt, 1 reciprocal, 2 1/sqrt (the fast correctly rounded
  * forms the cellular nodes use for positive normal arguments), 3 the IEEE
- * 1/sqrt. Lets a test compare them with IEEE results on the installed GPU. */
+ * 1/sqrt, 4 the bare hardware square root estimate, 5 the square root that
+ * corrects that estimate upward only (3D / 4D cellular jitter lengths). Lets
+ * a test compare them with IEEE results on the installed GPU. */
 FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n );
 
 /* Run-time specialisation of trees (hipRTC): a tree used repeatedly gets a

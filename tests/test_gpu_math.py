@@ -48,3 +48,27 @@ def test_fast_math_is_correctly_rounded(gpu, op, lo, hi):
         bad += n
     assert bad == 0, "%d mismatches, first (x, got, ieee) = %r" % (bad, first)
 
+
+
+def cell_lengths(dims):
+    """Every squared length a D-dimensional cellular jitter vector can have
+    (and more): its components are odd / 2 with |odd| <= 2^b - 1 (b = 10 in
+    3D, 8 in 4D), so the length is N / 4 for a sum N of D odd squares, and
+    N = D (mod 8); all such N up to the largest possible sum."""
+    top = {3: 1023, 4: 255}[dims]
+    return (np.arange(dims, dims * top * top + 1, 8, dtype=np.float64) / 4).astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [3, 4])
+def test_upward_corrected_sqrt_exact_on_cell_lengths(gpu, dims):
+    # the 3D / 4D cellular nodes correct the hardware square root estimate
+    # upward only; on every argument they can pass that must be the
+    # correctly rounded root
+    x = cell_lengths(dims)
+    assert np.array_equal(x.astype(np.float64) * 4, np.arange(dims, dims * {3: 1023, 4: 255}[dims] ** 2 + 1, 8))
+    din = fn.DeviceArray(x.size)
+    dout = fn.DeviceArray(x.size)
+    got = run(5, x, din, dout)
+    ref = np.sqrt(x)
+    bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
+    assert bad.size == 0, "%d mismatches, first at x = %r" % (bad.size, float(x[bad[0]]))
