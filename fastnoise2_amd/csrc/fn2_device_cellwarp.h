@@ -216,7 +216,9 @@ namespace fn2d
             int o[D];
 #pragma unroll
             for( int a = 0; a < D; a++ ) o[a] = ( code >> ( 2 * a ) ) & 3;
-            if( cls > 0 )
+            // after the centre alone, slot 1 and up are still +inf: with more
+            // than one slot no face cell can be left out, so faces are not tested
+            if( cls > 1 || ( cls == 1 && SLOTS == 1 ) )
             {
                 if( ( order.e[k - 1] >> 8 ) != cls )
                 {
