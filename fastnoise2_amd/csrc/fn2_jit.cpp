@@ -86,6 +86,16 @@ namespace fn2
             return -1;
         }
 
+        // Whether the program holds a CellularDistance that takes the pruned
+        // cell walk (squared distance, no size jitter): such programs walk grid
+        // requests in bricks, which lets whole waves leave out more cells
+        bool PrunedCellular( const DeviceProgram& p )
+        {
+            for( const Ins& in : p.ins )
+                if( in.op == OP_CELL_DISTANCE && ( in.p[0] < 2 || in.p[0] > 5 ) && in.r[3] == kNoReg ) return true;
+            return false;
+        }
+
         // Each instruction becomes an `execIns` instantiated for its own op,
         // dimensionality and (fused fractal) leaf op, so the compiler sees only
         // the code that instruction runs.
@@ -97,6 +107,7 @@ namespace fn2
                  "namespace {\nstruct Prog {\n"
                  "static constexpr bool kFresh = true;\n"
                  "static constexpr int kDims = " << p.dims << ";\n"
+                 "static constexpr bool kBricks = " << ( PrunedCellular( p ) ? "true" : "false" ) << ";\n"
                  "__device__ __forceinline__ float operator()( const fn2::GenArgs& a, float* R ) const {\n";
             for( const Ins& in : p.ins )
             {

@@ -68,12 +68,61 @@ namespace fn2k
         }
     };
 
+    // A grid request walked in wave-sized bricks (8 x 8 in 2D, 4 x 4 x 4 in
+    // 3D) instead of runs of 64 along x: the lanes of a wave then sit close
+    // together in every axis, so they agree more often on which cells a pruned
+    // cellular walk may leave out. Positions are computed as the row-major
+    // walk computes them; each lane writes its own output element.
+    template<int BLOCK, int D, class Prog>
+    __device__ __forceinline__ void generateBricks( const fn2::GenArgs& a, Prog prog, float& mn, float& mx )
+    {
+        constexpr uint32_t BX = D == 2 ? 8 : 4, BY = D == 2 ? 8 : 4, BZ = D == 2 ? 1 : 4;
+        const uint32_t nx = a.div[0].d;
+        const uint32_t ny = D == 2 ? (uint32_t)a.total / nx : a.div[1].d;
+        const uint32_t nz = D == 2 ? 1u : (uint32_t)a.total / ( nx * ny );
+        const uint32_t tx = ( nx + BX - 1 ) / BX, ty = ( ny + BY - 1 ) / BY, tz = ( nz + BZ - 1 ) / BZ;
+        const uint32_t bricks = tx * ty * tz;
+        const uint32_t lane = threadIdx.x & 63;
+        const uint32_t lx = lane % BX, ly = ( lane / BX ) % BY, lz = lane / ( BX * BY );
+        constexpr uint32_t WAVES = BLOCK / 64;
+        // the brick index is the same on every lane of a wave: scalar arithmetic
+        const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane( (int)( blockIdx.x * WAVES + threadIdx.x / 64 ) );
+        for( uint32_t t = first; t < bricks; t += gridDim.x * WAVES )
+        {
+            const uint32_t ix = t % tx, rest = t / tx, iy = rest % ty, iz = rest / ty;
+            const uint32_t x = ix * BX + lx, y = iy * BY + ly, z = iz * BZ + lz;
+            if( x < nx && y < ny && z < nz )
+            {
+                RegFile<Prog> fresh;
+                fresh.R[0] = fmadd( (float)(int32_t)x, a.step[0], a.offset[0] );
+                fresh.R[1] = fmadd( (float)(int32_t)y, a.step[1], a.offset[1] );
+                if constexpr( D == 3 ) fresh.R[2] = fmadd( (float)(int32_t)z, a.step[2], a.offset[2] );
+                const float v = prog( a, fresh.R );
+                a.out[( z * ny + y ) * nx + x] = v;
+                if( v == v )
+                {
+                    mn = fminf( mn, v );
+                    mx = fmaxf( mx, v );
+                }
+            }
+        }
+    }
+
     // One request: grid-stride loop over all of its points.
     template<int BLOCK, class Prog>
     __device__ __forceinline__ void generateSingle( const fn2::GenArgs& a, Prog prog )
     {
         RegFile<Prog> file;
         float mn = __builtin_inff(), mx = -__builtin_inff();
+        if constexpr( Prog::kBricks && ( Prog::kDims == 2 || Prog::kDims == 3 ) )
+        {
+            if( a.kind == fn2::SRC_GRID && a.dims == Prog::kDims )
+            {
+                generateBricks<BLOCK, Prog::kDims>( a, prog, mn, mx );
+                if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+                return;
+            }
+        }
         const uint32_t total = (uint32_t)a.total;
         const uint32_t stride = gridDim.x * BLOCK;
         for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )

@@ -27,6 +27,7 @@ struct Interpreted
 {
     static constexpr bool kFresh = false;   // indexes the register file at run time
     static constexpr int kDims = 0;         // a.dims at run time
+    static constexpr bool kBricks = false;  // row-major walk
     __device__ __forceinline__ float operator()( const fn2::GenArgs& g, float* R ) const { return runProgram<DM, FT>( g, R ); }
 };
 
