@@ -201,6 +201,8 @@ namespace fn2
                 }
                 const char* lim = getenv( "FN2_JIT_MAX_INSTRUCTIONS" );
                 if( lim && atoi( lim ) > 0 ) maxInstructions = atoi( lim );
+                const char* th = getenv( "FN2_JIT_THREADS" );
+                if( th && atoi( th ) > 0 ) threads = std::min( 16, atoi( th ) );
             }
 
             ~JitManager() { Shutdown(); }
@@ -215,7 +217,8 @@ namespace fn2
                     stop = true;
                 }
                 cv.notify_all();
-                if( worker.joinable() ) worker.join();
+                for( std::thread& w : workers )
+                    if( w.joinable() ) w.join();
             }
 
             int mode = FN2_JIT_ASYNC;
@@ -248,7 +251,9 @@ namespace fn2
                     {
                         if( stop ) return nullptr;
                         queue.push_back( p );
-                        if( !worker.joinable() ) worker = std::thread( [this] { Work(); } );
+                        // compilations are independent: a few run side by side
+                        if( (int)workers.size() < threads && (int)workers.size() < (int)queue.size() + busy )
+                            workers.emplace_back( [this] { Work(); } );
                         cv.notify_all();
                         return nullptr;
                     }
@@ -273,7 +278,7 @@ namespace fn2
             void Wait()
             {
                 std::unique_lock<std::mutex> lk( mu );
-                cv.wait( lk, [&] { return queue.empty() && !busy; } );
+                cv.wait( lk, [&] { return queue.empty() && busy == 0; } );
             }
 
         private:
@@ -288,14 +293,14 @@ namespace fn2
                         if( stop ) return;
                         p = queue.front();
                         queue.pop_front();
-                        busy = true;
+                        busy++;
                     }
                     Entry e;
                     e.state = Compile( *p, e.kernel, e.error ) ? kReady : kFailed;
                     {
                         std::lock_guard<std::mutex> lk( mu );
                         entries[p] = e;
-                        busy = false;
+                        busy--;
                     }
                     cv.notify_all();
                 }
@@ -305,8 +310,9 @@ namespace fn2
             std::condition_variable cv;
             std::unordered_map<const DeviceProgram*, Entry> entries;
             std::deque<const DeviceProgram*> queue;
-            std::thread worker;
-            bool busy = false;
+            std::vector<std::thread> workers;
+            int threads = 4;
+            int busy = 0;
             bool stop = false;
         };
 
