@@ -153,6 +153,40 @@ namespace fn2
         return f;
     }
 
+    // Registers operand k of an instruction spans: a vector operand covers
+    // the instruction's dimensionality (2 or 3 for the rotations), a scalar
+    // one register, an unused one none (the layouts of fn2_program.h)
+    int OperandExtent( const Ins& in, int k )
+    {
+        const int d = std::max<int>( 1, in.dim );
+        switch( in.op )
+        {
+        case OP_PERLIN: case OP_VALUE: case OP_SIMPLEX: case OP_SUPERSIMPLEX:
+        case OP_WHITE: case OP_CHECKERBOARD: case OP_SINEWAVE: case OP_FRACTAL_LEAF:
+            return k == 0 ? 1 : k == 1 ? d : 0;
+        case OP_CELL_VALUE: case OP_CELL_DISTANCE:
+            return k == 1 ? d : k <= 3 ? 1 : 0;
+        case OP_CELL_LOOKUP:
+            return k <= 1 ? d : k <= 3 ? 1 : 0;
+        case OP_WARP_GRADIENT: case OP_WARP_SIMPLEX: case OP_WARP_SUPERSIMPLEX:
+            return k == 1 || k == 2 ? d : k <= 3 ? 1 : 0;
+        case OP_CONST: return k == 0 ? 1 : 0;
+        case OP_MOV: case OP_ABS: case OP_POWI: case OP_SSQRT: case OP_RGBA8: return k <= 1 ? 1 : 0;
+        case OP_ADD: case OP_SUB: case OP_MUL: case OP_DIV: case OP_MIN: case OP_MAX: case OP_FMOD:
+        case OP_FBM_SUM: case OP_RIDGED_TERM: case OP_PINGPONG: case OP_POWF: case OP_TERRACE: case OP_GRADIENT_AXIS:
+            return k <= 2 ? 1 : 0;
+        case OP_FBM_AMP: case OP_RIDGED_AMP: case OP_WARP_AMP: case OP_MINSMOOTH: return k <= 3 ? 1 : 0;
+        case OP_FADE: case OP_REMAP: return 1;
+        case OP_SCALE: case OP_SCALE_AXES: case OP_MOV_VEC: return k <= 1 ? d : 0;
+        case OP_MUL_VEC: return k <= 1 ? d : k == 2 ? 1 : 0;
+        case OP_ADD_VEC: case OP_SUB_VEC: return k <= 2 ? d : 0;
+        case OP_ROTATE2: return k <= 1 ? 2 : 0;
+        case OP_ROTATE3: case OP_ROTATE_PLANE: return k <= 1 ? 3 : 0;
+        case OP_DISTANCE: return k == 1 ? d : k == 0 || k == 2 ? 1 : 0;
+        default: return 4;
+        }
+    }
+
     const DeviceProgram* GetDeviceProgram( const Program& p )
     {
         std::lock_guard<std::mutex> lk( gMu );
@@ -171,8 +205,16 @@ namespace fn2
             SetError( "device program cache full (more than 8192 distinct compiled trees)" );
             return nullptr;
         }
-        // every register operand must lie inside the register file and every
-        // constant-pool window (at most 16 floats) inside the uploaded pool
+        // every register an instruction touches (vector operands span dim
+        // registers) must lie below the program's register count, itself at
+        // most the register file; every constant-pool window (at most 16
+        // floats) must lie inside the uploaded pool
+        const int regs = std::max( 1, std::max( p.regsUsed, p.dims ) );
+        if( regs > kMaxRegs )
+        {
+            SetError( "compiled program uses more registers than the register file holds" );
+            return nullptr;
+        }
         for( const Ins& in : p.ins )
         {
             if( in.op == OP_END || in.op >= OP_COUNT || in.dim > 4 )
@@ -182,9 +224,10 @@ namespace fn2
             }
             for( int r = 0; r < 6; r++ )
             {
-                if( in.r[r] != kNoReg && in.r[r] >= kMaxRegs )
+                const int ext = OperandExtent( in, r );
+                if( in.r[r] != kNoReg && ext > 0 && in.r[r] + ext > regs )
                 {
-                    SetError( "compiled program addresses a register outside the register file" );
+                    SetError( "compiled program addresses a register outside its register file" );
                     return nullptr;
                 }
             }
@@ -206,6 +249,7 @@ namespace fn2
         d->dimMask = p.dimMask;
         d->featMask = p.featMask;
         d->dims = p.dims;
+        d->regs = regs;
         size_t insBytes = std::max<size_t>( 1, p.ins.size() ) * sizeof( Ins );
         // padded so that a window starting at the last constant stays in bounds
         size_t kBytes = ( p.konst.size() + 16 ) * sizeof( float );
@@ -471,6 +515,7 @@ namespace fn2
 
         a.ins = dp->dIns;
         a.konst = dp->dKonst;
+        a.regs = dp->regs;
         a.nIns = (int32_t)dp->ins.size();
         a.resultReg = dp->resultReg;
         a.minmax = c->dKeys;
@@ -743,6 +788,7 @@ FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, f
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     a.ins = dp->dIns;
     a.konst = dp->dKonst;
+    a.regs = dp->regs;
     a.nIns = (int32_t)dp->ins.size();
     a.resultReg = dp->resultReg;
     a.out = dOut;
@@ -789,6 +835,7 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
     struct Releaser { CallContext* c; ~Releaser() { ReleaseContext( c ); } } rel{ c };
     a.ins = dp->dIns;
     a.konst = dp->dKonst;
+    a.regs = dp->regs;
     a.nIns = (int32_t)dp->ins.size();
     a.resultReg = dp->resultReg;
     a.out = dOut;

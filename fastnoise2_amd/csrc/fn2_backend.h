@@ -21,6 +21,7 @@ namespace fn2
         int dimMask = 0;
         int featMask = 0;
         int dims = 0;    // of the starting positions (programs are cached per dimensionality)
+        int regs = 1;    // registers used: every operand lies below this
         Ins* dIns = nullptr;
         float* dKonst = nullptr;
     };

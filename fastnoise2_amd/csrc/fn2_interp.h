@@ -30,6 +30,15 @@ namespace fn2i
         return ( t1 + ( ( n - t1 ) >> d.s1 ) ) >> d.s2;
     }
 
+    // A register file whose registers lie STRIDE floats apart (one column of
+    // an LDS array shared by the block's threads); indexes like a float*.
+    template<int STRIDE>
+    struct StridedRegs
+    {
+        float* base;
+        FN2D float& operator[]( int i ) const { return base[i * STRIDE]; }
+    };
+
     // order-preserving int key of a float (for atomicMin / atomicMax)
     FN2D int32_t orderedKey( float f )
     {
@@ -77,8 +86,8 @@ namespace fn2i
         return 0.0f;
     }
 
-    template<int D, int OP = -1>
-    FN2D void leafAt( const Ins& in, const float* K, int32_t seed, const float* R, int p, float& out )
+    template<int D, int OP = -1, class RF = const float*>
+    FN2D void leafAt( const Ins& in, const float* K, int32_t seed, RF R, int p, float& out )
     {
         const float freq = K[0];
         float x[D];
@@ -89,8 +98,8 @@ namespace fn2i
 
     // FractalFBm / FractalRidged over a lattice leaf with constant gain and
     // weighted strength (Fractal.inl:9-67); the octave loop stays in registers.
-    template<int D, int LEAF = -1>
-    FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, const float* R, int p )
+    template<int D, int LEAF = -1, class RF = const float*>
+    FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, RF R, int p )
     {
         const int leaf = in.p[0], octaves = in.p[1];
         const bool ridged = in.p[2] != 0;
@@ -129,8 +138,8 @@ namespace fn2i
 
     // SLOTS: distance slots CellularDistance keeps (4, or fewer when the
     // instruction's distance indices are compile-time constants)
-    template<int D, int OP = -1, int SLOTS = 4>
-    FN2D void cellular( const Ins& in, const float* K, int32_t seed, float* R )
+    template<int D, int OP = -1, int SLOTS = 4, class RF = float*>
+    FN2D void cellular( const Ins& in, const float* K, int32_t seed, RF R )
     {
         CellParams cp;
         cp.seed = seed;
@@ -168,8 +177,8 @@ namespace fn2i
         }
     }
 
-    template<int D, int OP = -1>
-    FN2D void warpOp( const Ins& in, const float* K, int32_t seed, float* R )
+    template<int D, int OP = -1, class RF = float*>
+    FN2D void warpOp( const Ins& in, const float* K, int32_t seed, RF R )
     {
         float sp[D], out[D], axis[D];
 #pragma unroll
@@ -187,8 +196,8 @@ namespace fn2i
         if( in.r[0] != fn2::kNoReg ) R[in.r[0]] = s;
     }
 
-    template<int D>
-    FN2D void distanceOp( const Ins& in, float* R )
+    template<int D, class RF>
+    FN2D void distanceOp( const Ins& in, RF R )
     {
         float d[D];
 #pragma unroll
@@ -208,8 +217,8 @@ namespace fn2i
     // instruction's op and, for OP_FRACTAL_LEAF, its leaf op, for
     // OP_CELL_DISTANCE the number of distance slots it reads, so that only
     // the code the instruction runs is instantiated.
-    template<int DM, int FT, int OP = -1, int AUX = -1>
-    FN2D void execIns( const Ins& in, const float* K, const int32_t seed, float* R )
+    template<int DM, int FT, int OP = -1, int AUX = -1, class RF = float*>
+    FN2D void execIns( const Ins& in, const float* K, const int32_t seed, RF R )
     {
         {
             const int d = in.dim;
@@ -488,8 +497,8 @@ namespace fn2i
         }
     }
 
-    template<int DM, int FT>
-    FN2D float runProgram( const fn2::GenArgs& a, float* R )
+    template<int DM, int FT, class RF>
+    FN2D float runProgram( const fn2::GenArgs& a, RF R )
     {
         for( int pc = 0; pc < a.nIns; pc++ )
         {
@@ -501,7 +510,8 @@ namespace fn2i
 
     // Starting position of point `idx` into R[0..dims): grid, position arrays
     // or the tileable torus (Generator.inl:79-433).
-    FN2D void startPosition( const fn2::GenArgs& a, uint32_t idx, float* R )
+    template<class RF>
+    FN2D void startPosition( const fn2::GenArgs& a, uint32_t idx, RF R )
     {
         if( a.kind == fn2::SRC_POSITIONS )
         {
@@ -534,8 +544,8 @@ namespace fn2i
     // kernels): every source writes the same D registers, from values computed
     // per source, so the register file never needs a run-time index. A
     // tileable call always runs a 4D program.
-    template<int D>
-    FN2D void startPositionD( const fn2::GenArgs& a, uint32_t idx, float* R )
+    template<int D, class RF>
+    FN2D void startPositionD( const fn2::GenArgs& a, uint32_t idx, RF R )
     {
         float p[D];
         if( a.kind == fn2::SRC_POSITIONS )

@@ -24,6 +24,6 @@ namespace fn2
         float step[4];
         const float* pos[4];     // position arrays (device); tileable: cos x, cos y, sin x, sin y tables
         int32_t ppt;             // batched kernel: points per lane of each block of this job
-        int32_t reserved;
+        int32_t regs;            // registers the program uses (the interpreter's file holds as many)
     };
 }

@@ -50,23 +50,27 @@ namespace fn2k
         }
     }
 
-    // The register file of one point. Interpreted programs index it at run
-    // time, so it is zeroed once per thread; a specialised program (whose
-    // every register is written before it is read) gets a fresh, unset file
-    // per point, which leaves the compiler no values to carry between points.
-    template<class Prog>
+    // The register file of one point of a specialised program (every register
+    // written before it is read, all indices compile-time constants): a fresh
+    // private array per point, which lives in VGPRs and leaves the compiler no
+    // values to carry between points.
     struct RegFile
     {
         float R[fn2::kMaxRegs];
-        __device__ __forceinline__ RegFile()
-        {
-            if constexpr( !Prog::kFresh )
-            {
-#pragma unroll
-                for( int r = 0; r < fn2::kMaxRegs; r++ ) R[r] = 0.0f;
-            }
-        }
     };
+
+    // The interpreter's register files: indexed at run time, so they live in
+    // LDS, one column per thread (consecutive threads in consecutive banks),
+    // zeroed once. The launch sizes the LDS to the program's register count,
+    // so short programs keep the occupancy their VGPRs allow. Neither VGPR
+    // indexing nor scratch memory, whose spills reach HBM on long programs.
+    template<int BLOCK>
+    __device__ __forceinline__ StridedRegs<BLOCK> ldsRegs( int regs )
+    {
+        extern __shared__ float fn2RegisterFile[];   // regs * BLOCK floats
+        for( int r = 0; r < regs; r++ ) fn2RegisterFile[r * BLOCK + threadIdx.x] = 0.0f;
+        return StridedRegs<BLOCK>{ fn2RegisterFile + threadIdx.x };
+    }
 
     // A grid request walked in wave-sized bricks (8 x 8 in 2D, 4 x 4 x 4 in
     // 3D) instead of runs of 64 along x: the lanes of a wave then sit close
@@ -93,7 +97,7 @@ namespace fn2k
             const uint32_t x = ix * BX + lx, y = iy * BY + ly, z = iz * BZ + lz;
             if( x < nx && y < ny && z < nz )
             {
-                RegFile<Prog> fresh;
+                RegFile fresh;
                 fresh.R[0] = fmadd( (float)(int32_t)x, a.step[0], a.offset[0] );
                 fresh.R[1] = fmadd( (float)(int32_t)y, a.step[1], a.offset[1] );
                 if constexpr( D == 3 ) fresh.R[2] = fmadd( (float)(int32_t)z, a.step[2], a.offset[2] );
@@ -112,38 +116,45 @@ namespace fn2k
     template<int BLOCK, class Prog>
     __device__ __forceinline__ void generateSingle( const fn2::GenArgs& a, Prog prog )
     {
-        RegFile<Prog> file;
         float mn = __builtin_inff(), mx = -__builtin_inff();
-        if constexpr( Prog::kBricks && ( Prog::kDims == 2 || Prog::kDims == 3 ) )
-        {
-            if( a.kind == fn2::SRC_GRID && a.dims == Prog::kDims )
-            {
-                generateBricks<BLOCK, Prog::kDims>( a, prog, mn, mx );
-                if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
-                return;
-            }
-        }
-        const uint32_t total = (uint32_t)a.total;
-        const uint32_t stride = gridDim.x * BLOCK;
-        for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
-        {
-            float v;
-            if constexpr( Prog::kFresh )
-            {
-                RegFile<Prog> fresh;
-                startPositionD<Prog::kDims>( a, i, fresh.R );
-                v = prog( a, fresh.R );
-            }
-            else
-            {
-                startPosition( a, i, file.R );
-                v = prog( a, file.R );
-            }
+        auto emit = [&]( uint32_t i, float v ) {
             a.out[i] = v;
             if( v == v )
             {
                 mn = fminf( mn, v );
                 mx = fmaxf( mx, v );
+            }
+        };
+        const uint32_t total = (uint32_t)a.total;
+        const uint32_t stride = gridDim.x * BLOCK;
+        if constexpr( Prog::kFresh )
+        {
+            bool bricks = false;
+            if constexpr( Prog::kBricks && ( Prog::kDims == 2 || Prog::kDims == 3 ) )
+            {
+                if( a.kind == fn2::SRC_GRID && a.dims == Prog::kDims )
+                {
+                    generateBricks<BLOCK, Prog::kDims>( a, prog, mn, mx );
+                    bricks = true;
+                }
+            }
+            if( !bricks )
+            {
+                for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
+                {
+                    RegFile fresh;
+                    startPositionD<Prog::kDims>( a, i, fresh.R );
+                    emit( i, prog( a, fresh.R ) );
+                }
+            }
+        }
+        else
+        {
+            const auto R = ldsRegs<BLOCK>( a.regs );
+            for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
+            {
+                startPosition( a, i, R );
+                emit( i, prog( a, R ) );
             }
         }
         if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
@@ -159,34 +170,43 @@ namespace fn2k
         // the job is read through a pointer (uniform address -> scalar loads)
         // rather than copied into a private struct
         const fn2::GenArgs& a = jobs[blockJob[blockIdx.x]];
-        RegFile<Prog> file;
         float mn = __builtin_inff(), mx = -__builtin_inff();
+        auto emit = [&]( uint32_t i, float v ) {
+            a.out[i] = v;
+            if( v == v )
+            {
+                mn = fminf( mn, v );
+                mx = fmaxf( mx, v );
+            }
+        };
         const uint32_t total = (uint32_t)a.total;
         const uint32_t base = blockBase[blockIdx.x] + threadIdx.x;
         const int ppt = a.ppt;
-#pragma unroll 1
-        for( int p = 0; p < ppt; p++ )
+        if constexpr( Prog::kFresh )
         {
-            const uint32_t i = base + p * BLOCK;
-            if( i < total )
+#pragma unroll 1
+            for( int p = 0; p < ppt; p++ )
             {
-                float v;
-                if constexpr( Prog::kFresh )
+                const uint32_t i = base + p * BLOCK;
+                if( i < total )
                 {
-                    RegFile<Prog> fresh;
+                    RegFile fresh;
                     startPositionD<Prog::kDims>( a, i, fresh.R );
-                    v = prog( a, fresh.R );
+                    emit( i, prog( a, fresh.R ) );
                 }
-                else
+            }
+        }
+        else
+        {
+            const auto R = ldsRegs<BLOCK>( a.regs );
+#pragma unroll 1
+            for( int p = 0; p < ppt; p++ )
+            {
+                const uint32_t i = base + p * BLOCK;
+                if( i < total )
                 {
-                    startPosition( a, i, file.R );
-                    v = prog( a, file.R );
-                }
-                a.out[i] = v;
-                if( v == v )
-                {
-                    mn = fminf( mn, v );
-                    mx = fmaxf( mx, v );
+                    startPosition( a, i, R );
+                    emit( i, prog( a, R ) );
                 }
             }
         }

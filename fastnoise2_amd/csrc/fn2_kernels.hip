@@ -25,10 +25,11 @@ using namespace fn2i;
 template<int DM, int FT>
 struct Interpreted
 {
-    static constexpr bool kFresh = false;   // indexes the register file at run time
+    static constexpr bool kFresh = false;   // indexes the register file at run time (kept in LDS)
     static constexpr int kDims = 0;         // a.dims at run time
     static constexpr bool kBricks = false;  // row-major walk
-    __device__ __forceinline__ float operator()( const fn2::GenArgs& g, float* R ) const { return runProgram<DM, FT>( g, R ); }
+    template<class RF>
+    __device__ __forceinline__ float operator()( const fn2::GenArgs& g, RF R ) const { return runProgram<DM, FT>( g, R ); }
 };
 
 // Single call of the interpreter: grid-stride loop over all points of one request.
@@ -160,11 +161,14 @@ int fn2_batch_block_threads( void )
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream )
 {
     if( args->total <= 0 ) return 0;
+    // the register file in LDS: one column of args->regs floats per thread
+    if( args->regs < 1 || args->regs > fn2::kMaxRegs ) return (int)hipErrorInvalidValue;
+    const size_t lds = sizeof( float ) * args->regs * kBlock;
     unsigned grid = GridFor( args->total );
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        hipLaunchKernelGGL( ( fn2_generate<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), 0, stream, *args ); \
+        hipLaunchKernelGGL( ( fn2_generate<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), lds, stream, *args ); \
         return (int)hipGetLastError(); \
     }
     FN2_VARIANTS( FN2_TRY )
@@ -172,13 +176,17 @@ int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hi
     return (int)hipErrorInvalidValue;
 }
 
-int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask, hipStream_t stream )
+// regs: the largest register count of the jobs in the launch
+int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask,
+                               int regs, hipStream_t stream )
 {
     if( nBlocks <= 0 ) return 0;
+    if( regs < 1 || regs > fn2::kMaxRegs ) return (int)hipErrorInvalidValue;
+    const size_t lds = sizeof( float ) * regs * kBlock;
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        hipLaunchKernelGGL( ( fn2_generate_batch<kBlock, DMV, FTV> ), dim3( nBlocks ), dim3( kBlock ), 0, stream, dJobs, dBlockJob, dBlockBase ); \
+        hipLaunchKernelGGL( ( fn2_generate_batch<kBlock, DMV, FTV> ), dim3( nBlocks ), dim3( kBlock ), lds, stream, dJobs, dBlockJob, dBlockBase ); \
         return (int)hipGetLastError(); \
     }
     FN2_VARIANTS( FN2_TRY )

@@ -25,6 +25,7 @@
 // and variant groups); job records and tables change, the topology does not.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -44,7 +45,8 @@
 
 extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
-int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask, hipStream_t stream );
+int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask,
+                               int regs, hipStream_t stream );
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
 int fn2_batch_block_threads( void );
 }
@@ -114,7 +116,7 @@ namespace
         // graph mode
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        std::vector<std::pair<int64_t, int>> graphShape;   // (kernel key, blocks) per launch
+        std::vector<std::array<int64_t, 3>> graphShape;   // (kernel key, blocks, register file) per launch
         int graphJobs = -1;
     };
 
@@ -407,6 +409,7 @@ namespace
             r->prog = prog;
             r->args.ins = prog->dIns;
             r->args.konst = prog->dKonst;
+            r->args.regs = prog->regs;
             r->args.nIns = (int32_t)prog->ins.size();
             r->args.resultReg = prog->resultReg;
             r->tSubmit = Clock::now();
@@ -505,6 +508,7 @@ namespace
             }
             a.ins = prog->dIns;
             a.konst = prog->dKonst;
+            a.regs = prog->regs;
             a.nIns = (int32_t)prog->ins.size();
             a.resultReg = prog->resultReg;
             a.out = tuneOut;
@@ -533,7 +537,7 @@ namespace
                 {
                     if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, tuneStream ), "tune launch" ) ||
                         !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
-                        !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, tuneStream ), "tune launch" ) ||
+                        !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ), "tune launch" ) ||
                         !fn2::HipOk( hipEventRecord( tuneEv1, tuneStream ), "tune event" ) ||
                         !fn2::HipOk( hipEventSynchronize( tuneEv1 ), "tune run" ) )
                         return -1;
@@ -616,7 +620,9 @@ namespace
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             int32_t* hBJ = (int32_t*)( s.hTables + jobsBytes );
             uint32_t* hBB = (uint32_t*)( s.hTables + jobsBytes + bjBytes );
-            std::vector<std::pair<int64_t, int>> shape;   // (kernel key, blocks)
+            // (kernel key, blocks, register-file size) per launch; the register
+            // file (interpreter launches only) sizes the launch's LDS
+            std::vector<std::array<int64_t, 3>> shape;
             std::vector<const fn2::JitKernel*> kernels;   // per entry of shape
             // A batch too small to fill the GPU at the tuned points per lane
             // runs with fewer points per lane (more blocks): the tuned value
@@ -642,12 +648,17 @@ namespace
                     hBB[nb + b] = (uint32_t)( b * tile );
                 }
                 int64_t key = KernelKey( r->prog, r->jit );
-                if( shape.empty() || shape.back().first != key )
+                const int64_t regs = r->jit ? 0 : r->prog->regs;
+                if( shape.empty() || shape.back()[0] != key )
                 {
-                    shape.emplace_back( key, rb );
+                    shape.push_back( { key, rb, regs } );
                     kernels.push_back( r->jit );
                 }
-                else shape.back().second += rb;
+                else
+                {
+                    shape.back()[1] += rb;
+                    shape.back()[2] = std::max( shape.back()[2], regs );
+                }
                 nb += rb;
             }
             const int nJobs = (int)order.size();
@@ -685,7 +696,7 @@ namespace
         // Stream work of one batch that depends only on its shape (recorded
         // once per shape in graph mode): table upload, range-key reset, one
         // batch launch per kernel.
-        bool Enqueue( Slot& s, const std::vector<std::pair<int64_t, int>>& shape, const std::vector<const fn2::JitKernel*>& kernels,
+        bool Enqueue( Slot& s, const std::vector<std::array<int64_t, 3>>& shape, const std::vector<const fn2::JitKernel*>& kernels,
                       int nJobs, int nb )
         {
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
@@ -705,16 +716,17 @@ namespace
                 {
                     const fn2::GenArgs* jobs = s.dJobs;
                     void* params[] = { (void*)&jobs, (void*)&bj, (void*)&bb };
-                    if( !fn2::HipOk( hipModuleLaunchKernel( kernels[gi]->batch, g.second, 1, 1, kThreads, 1, 1, 0, s.stream, params, nullptr ),
+                    if( !fn2::HipOk( hipModuleLaunchKernel( kernels[gi]->batch, (unsigned)g[1], 1, 1, kThreads, 1, 1, 0, s.stream, params, nullptr ),
                                      "specialised batch launch" ) )
                         return false;
                 }
                 else
                 {
-                    int dm = (int)( g.first / 16 ), fm = (int)( g.first % 16 );
-                    if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, bj, bb, g.second, dm, fm, s.stream ), "batch launch" ) ) return false;
+                    int dm = (int)( g[0] / 16 ), fm = (int)( g[0] % 16 );
+                    if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.stream ), "batch launch" ) )
+                        return false;
                 }
-                first += g.second;
+                first += (int)g[1];
             }
             return true;
         }
