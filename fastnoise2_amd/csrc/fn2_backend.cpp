@@ -29,7 +29,7 @@
 
 extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
-unsigned fn2_grid_for( int64_t total );
+unsigned fn2_grid_for( int64_t total, int blocksPerCU );
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
 int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStream_t stream );
 }
@@ -133,7 +133,7 @@ namespace fn2
             if( specialised ) *specialised = true;
             GenArgs copy = a;
             void* params[] = { &copy };
-            return hipModuleLaunchKernel( k->single, fn2_grid_for( a.total ), 1, 1, 256, 1, 1, 0, stream, params, nullptr );
+            return hipModuleLaunchKernel( k->single, fn2_grid_for( a.total, k->blocksPerCU ), 1, 1, 256, 1, 1, 0, stream, params, nullptr );
         }
         if( specialised ) *specialised = false;
         return (hipError_t)fn2_launch_generate( &a, dp->dimMask, dp->featMask, stream );

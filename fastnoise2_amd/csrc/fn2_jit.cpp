@@ -185,6 +185,11 @@ namespace fn2
                 return false;
             }
             k.module = mod;
+            if( hipModuleOccupancyMaxActiveBlocksPerMultiprocessor( &k.blocksPerCU, k.single, 256, 0 ) != hipSuccess )
+            {
+                (void)hipGetLastError();
+                k.blocksPerCU = 0;
+            }
             return true;
         }
 

@@ -12,6 +12,7 @@ namespace fn2
         hipModule_t module = nullptr;
         hipFunction_t single = nullptr;   // fn2_jit_single( GenArgs )
         hipFunction_t batch = nullptr;    // fn2_jit_batch( jobs, blockJob, blockBase )
+        int blocksPerCU = 0;              // resident 256-thread blocks of `single` per CU
     };
 
     // The specialised kernel of a program when it is loaded, else null (and,

@@ -15,6 +15,8 @@
 // kernel gives each block a tile of several points per lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
+#include <cstdlib>
 #include "fn2_kernel_body.h"
 #include "fn2_kernel_args.h"
 
@@ -113,9 +115,7 @@ namespace
 {
     constexpr int kBlock = 256;
 
-    // Blocks of the single-call kernel: enough to fill every CU several times
-    // over, few enough that the per-block range atomics stay negligible.
-    unsigned GridFor( int64_t total )
+    int CUs()
     {
         static int cus = 0;
         if( cus == 0 )
@@ -128,9 +128,40 @@ namespace
             }
             cus = n;
         }
-        int64_t need = ( total + kBlock - 1 ) / kBlock;
-        int64_t cap = (int64_t)cus * 16;
+        return cus;
+    }
+
+    // Blocks of a single-call (grid-stride) kernel. With the kernel's resident
+    // blocks per CU known: `rounds` times what the GPU holds at once (measured
+    // on MI355X: 2 for specialised kernels, 4 for the interpreter; more blocks
+    // cost more range atomics), but never fewer than 2 points per thread, since
+    // the per-thread prologue and the block's range reduction cost about as much
+    // as a point (a 1 M-point 4D call runs 1.8x faster at 2 points per thread
+    // than at 1). Without it, 16 blocks per CU.
+    unsigned GridFor( int64_t total, int blocksPerCU = 0, int rounds = 2 )
+    {
+        if( blocksPerCU <= 0 )
+        {
+            const int64_t need = ( total + kBlock - 1 ) / kBlock;
+            const int64_t cap = (int64_t)CUs() * 16;
+            return (unsigned)( need < cap ? need : cap );
+        }
+        const int64_t need = std::max<int64_t>( 1, ( total + 2 * kBlock - 1 ) / ( 2 * kBlock ) );
+        const int64_t cap = (int64_t)CUs() * blocksPerCU * rounds;
         return (unsigned)( need < cap ? need : cap );
+    }
+
+    // Resident blocks per CU of an interpreter variant at a register-file size
+    template<class K>
+    int ResidentBlocks( K kernel, size_t lds )
+    {
+        int n = 0;
+        if( hipOccupancyMaxActiveBlocksPerMultiprocessor( &n, kernel, kBlock, lds ) != hipSuccess )
+        {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return n;
     }
 }
 
@@ -146,9 +177,9 @@ extern "C" {
 
 // blocks of a single-call launch over `total` points (also used for the
 // run-time specialised kernels)
-unsigned fn2_grid_for( int64_t total )
+unsigned fn2_grid_for( int64_t total, int blocksPerCU )
 {
-    return GridFor( total );
+    return GridFor( total, blocksPerCU );
 }
 
 // threads per block of the batched kernel: a job with ppt points per lane
@@ -164,10 +195,12 @@ int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hi
     // the register file in LDS: one column of args->regs floats per thread
     if( args->regs < 1 || args->regs > fn2::kMaxRegs ) return (int)hipErrorInvalidValue;
     const size_t lds = sizeof( float ) * args->regs * kBlock;
-    unsigned grid = GridFor( args->total );
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
+        static int resident[fn2::kMaxRegs + 1] = {};   /* per register count */ \
+        if( resident[args->regs] == 0 ) resident[args->regs] = std::max( 1, ResidentBlocks( fn2_generate<kBlock, DMV, FTV>, lds ) ); \
+        const unsigned grid = GridFor( args->total, resident[args->regs], 4 ); \
         hipLaunchKernelGGL( ( fn2_generate<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), lds, stream, *args ); \
         return (int)hipGetLastError(); \
     }
