@@ -227,7 +227,7 @@ namespace fn2
             }
 
             int mode = FN2_JIT_ASYNC;
-            size_t maxInstructions = 64;
+            size_t maxInstructions = 160;
 
             const JitKernel* Get( const DeviceProgram* p )
             {
