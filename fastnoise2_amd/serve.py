@@ -1,6 +1,7 @@
 """Python face of the serving engine (fn2Serve* in include/fn2_gpu.h):
 continuous batching of chunk requests against resident node trees."""
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -18,9 +19,12 @@ class ServeEngine:
             raise RuntimeError("fn2ServeCreate failed: " + last_error())
         self.mode = mode
         self._outs = {}
+        self._clients = weakref.WeakSet()   # worlds built on this engine: closed first
 
     def close(self):
         if self.h:
+            for c in list(self._clients):
+                c.close()
             self.lib.fn2ServeDestroy(self.h)
             self.h = None
 

@@ -43,6 +43,7 @@ class ChunkWorld:
             raise RuntimeError("fn2WorldCreate failed: " + last_error())
         self.chunk_size = chunk_size
         self._buf = (ChunkResult * 256)()
+        engine._clients.add(self)   # the engine closes its worlds before itself
 
     def close(self):
         if self.h:
