@@ -405,7 +405,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "f32",
         "data": "synthetic: grid coordinates / SplitMix64(7) positions, output device-resident",
         "config": {"workload": args.workload, "tree": enc, "points": n, "seed": wl["seed"]},
         "jit_compile_s": round(jit_compile_s, 2) if args.jit == "sync" else None,
