@@ -175,15 +175,16 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=20000)
     the Poisson trace (open loop) for p50/p99 latency, and a saturating closed
     loop (512 outstanding) of the same request mix for throughput, into pinned
     host buffers and into device buffers, in the plain (one launch + copy per
-    chunk), batched and graph-replay modes."""
+    chunk), batched, graph-replay and persistent (one work-queue launch per
+    batch) modes."""
     import fastnoise2_amd as fn
-    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
+    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, SERVE_PERSISTENT
     enc = W.serving_tree().encode()
     specs, arrivals = W.serving_trace(trace_requests, rate)
     pts = np.array([s.count[0] * s.count[1] * (s.count[2] if s.dims == 3 else 1) for s in specs], dtype=np.float64)
     out = {"tree": enc, "trace": "Poisson %g req/s, %d requests, sizes 16^3/32^3/64^3/128^2/512^2" % (rate, trace_requests),
            "budget_bytes": 16 << 30}
-    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH)):
+    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH), ("persistent", SERVE_PERSISTENT)):
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tid = eng.add_tree(enc)
         eng.replay([tid] * 64, specs[:64], None, 16)   # compile + warm up
@@ -218,16 +219,17 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=20000)
 def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
     """The mixed-tree serving workload: 8 seeded random trees (depth 3..12),
     2D/3D/4D requests of 2^12..2^22 points, bursty on/off arrivals (10 ms at
-    10k req/s); per-tree tuned batched launches vs graph replay vs plain."""
+    10k req/s); per-tree tuned batched launches vs graph replay vs the
+    persistent work-queue kernel vs plain."""
     import fastnoise2_amd as fn
-    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
+    from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, SERVE_PERSISTENT
     encs = W.mixed_trees()
     trees, specs, arrivals = W.mixed_trace(trace_requests)
     pts = np.array([float(np.prod(list(s.count)[:s.dims])) for s in specs])
     out = {"trees": len(encs), "trace": "on/off bursts 10 ms at 10k req/s, %d requests, 2^12..2^22 points, dims 2-4"
            % trace_requests, "mean_points": round(float(pts.mean()))}
     warm = [W.grid_spec_like(d) for d in (2, 3, 4)]
-    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH)):
+    for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH), ("persistent", SERVE_PERSISTENT)):
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tids = [eng.add_tree(e) for e in encs]
         wt = [t for t in tids for _ in warm]

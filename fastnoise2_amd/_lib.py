@@ -38,7 +38,7 @@ class GenSpec(ctypes.Structure):
 
 
 SOURCE_GRID, SOURCE_POSITIONS, SOURCE_TILEABLE = 0, 1, 2
-SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH = 0, 1, 2
+SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, SERVE_PERSISTENT = 0, 1, 2, 3
 
 # name -> (restype, argtypes); the FastNoise_C.h entry points first.
 C_ABI = {

@@ -50,6 +50,21 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArg
     fn2k::generateBatch<BLOCK>( jobs, blockJob, blockBase, Interpreted<DM, FT>() );
 }
 
+// Persistent form for the serving layer: one launch of resident blocks per
+// batch pulls the batch's tiles (any tree, size and source) from a device
+// counter, which fn2_queue_reset zeroes on the same stream before the launch.
+template<int BLOCK, int DM, int FT>
+__global__ void __launch_bounds__( BLOCK ) fn2_generate_queue( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ tileJob,
+                                                              const uint32_t* __restrict__ tileBase, uint32_t nTiles, uint32_t* counter )
+{
+    fn2k::generateQueue<BLOCK>( jobs, tileJob, tileBase, nTiles, counter, Interpreted<DM, FT>() );
+}
+
+__global__ void fn2_queue_reset( uint32_t* counter )
+{
+    if( threadIdx.x == 0 ) *counter = 0;
+}
+
 __global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,6 +235,34 @@ int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJ
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
         hipLaunchKernelGGL( ( fn2_generate_batch<kBlock, DMV, FTV> ), dim3( nBlocks ), dim3( kBlock ), lds, stream, dJobs, dBlockJob, dBlockBase ); \
+        return (int)hipGetLastError(); \
+    }
+    FN2_VARIANTS( FN2_TRY )
+#undef FN2_TRY
+    return (int)hipErrorInvalidValue;
+}
+
+// The tiles of a batch through the work-queue kernel: a grid of at most the
+// resident blocks of the variant (every block is on a CU from the start and
+// pulls tiles until none is left). regs: the largest register count of the
+// jobs; counter: one device word, zeroed here on the stream first.
+int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
+                               int regs, uint32_t* dCounter, hipStream_t stream )
+{
+    if( nTiles <= 0 ) return 0;
+    if( regs < 1 || regs > fn2::kMaxRegs || !dCounter ) return (int)hipErrorInvalidValue;
+    const size_t lds = sizeof( float ) * regs * kBlock;
+#define FN2_TRY( DMV, FTV ) \
+    if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
+    { \
+        static int resident[fn2::kMaxRegs + 1] = {};   /* per register count */ \
+        if( resident[regs] == 0 ) resident[regs] = std::max( 1, ResidentBlocks( fn2_generate_queue<kBlock, DMV, FTV>, lds ) ); \
+        const unsigned grid = (unsigned)std::min<int64_t>( nTiles, (int64_t)CUs() * resident[regs] ); \
+        hipLaunchKernelGGL( fn2_queue_reset, dim3( 1 ), dim3( 64 ), 0, stream, dCounter ); \
+        hipError_t e = hipGetLastError(); \
+        if( e != hipSuccess ) return (int)e; \
+        hipLaunchKernelGGL( ( fn2_generate_queue<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), lds, stream, dJobs, dTileJob, dTileBase, \
+                            (uint32_t)nTiles, dCounter ); \
         return (int)hipGetLastError(); \
     }
     FN2_VARIANTS( FN2_TRY )

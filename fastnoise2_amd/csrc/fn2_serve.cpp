@@ -23,6 +23,9 @@
 // batch's table upload, launches and copies into a HIP graph and replays the
 // graph when the next batch has the same shape (same job count, block count
 // and variant groups); job records and tables change, the topology does not.
+// FN2_SERVE_PERSISTENT runs each batch, whatever its trees, as one launch of
+// the interpreter's work-queue kernel: a grid of resident blocks pulls the
+// batch's tiles from a device counter (specialised kernels are not used).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <array>
@@ -48,6 +51,8 @@ int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hi
 int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask,
                                int regs, hipStream_t stream );
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
+int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
+                               int regs, uint32_t* dCounter, hipStream_t stream );
 int fn2_batch_block_threads( void );
 }
 
@@ -103,7 +108,8 @@ namespace
         hipEvent_t done = nullptr;
         float* dOut = nullptr;          // capacity floats
         float* hOut = nullptr;          // pinned
-        int32_t* dKeys = nullptr;       // 2 per job
+        int32_t* dKeys = nullptr;       // 2 per job, then the tile counter of the persistent mode
+        uint32_t* dCounter = nullptr;
         int32_t* hKeys = nullptr;       // pinned
         GenArgs* dJobs = nullptr;
         int32_t* dBlockJob = nullptr;
@@ -232,7 +238,8 @@ namespace
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
-            size_t perSlot = (size_t)capacity * 4 + jobsBytes + bjBytes + bbBytes + 2 * sizeof( int32_t ) * kMaxJobs;
+            const size_t keyBytes = 2 * sizeof( int32_t ) * kMaxJobs + 256;
+            size_t perSlot = (size_t)capacity * 4 + jobsBytes + bjBytes + bbBytes + keyBytes;
             int nSlots = mode == FN2_SERVE_PLAIN ? 1 : kSlots;
             if( perSlot * nSlots > budget )
             {
@@ -247,7 +254,7 @@ namespace
                     !fn2::HipOk( hipEventCreateWithFlags( &s.done, hipEventDisableTiming ), "serve event" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dOut, (size_t)capacity * 4 ), "serve arena" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hOut, (size_t)capacity * 4, hipHostMallocDefault ), "serve staging" ) ||
-                    !fn2::HipOk( hipMalloc( (void**)&s.dKeys, 2 * sizeof( int32_t ) * kMaxJobs ), "serve keys" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&s.dKeys, keyBytes ), "serve keys" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, 2 * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dJobs, s.tableBytes ), "serve tables" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hTables, s.tableBytes, hipHostMallocDefault ), "serve tables" ) )
@@ -255,6 +262,7 @@ namespace
                     initError = fn2::LastError();
                     return false;
                 }
+                s.dCounter = (uint32_t*)( s.dKeys + 2 * kMaxJobs );
                 s.dBlockJob = (int32_t*)( (uint8_t*)s.dJobs + jobsBytes );
                 s.dBlockBase = (uint32_t*)( (uint8_t*)s.dJobs + jobsBytes + bjBytes );
             }
@@ -612,7 +620,7 @@ namespace
         {
             // jobs sorted by kernel so that each kernel's blocks are contiguous
             std::vector<Request*> order( s.reqs );
-            for( Request* r : order ) r->jit = fn2::GetJitKernel( r->prog );
+            for( Request* r : order ) r->jit = mode == FN2_SERVE_PERSISTENT ? nullptr : fn2::GetJitKernel( r->prog );
             std::stable_sort( order.begin(), order.end(),
                               []( Request* a, Request* b ) { return KernelKey( a->prog, a->jit ) < KernelKey( b->prog, b->jit ); } );
             GenArgs* hJobs = (GenArgs*)s.hTables;
@@ -660,6 +668,19 @@ namespace
                     shape.back()[2] = std::max( shape.back()[2], regs );
                 }
                 nb += rb;
+            }
+            if( mode == FN2_SERVE_PERSISTENT && shape.size() > 1 )
+            {
+                // one work-queue launch: the interpreter variant covering every job
+                int64_t dm = 0, fm = 0, regs = 0;
+                for( const auto& g : shape )
+                {
+                    dm |= g[0] / 16;
+                    fm |= g[0] % 16;
+                    regs = std::max( regs, g[2] );
+                }
+                shape.assign( 1, { dm * 16 + fm, nb, regs } );
+                kernels.assign( 1, nullptr );
             }
             const int nJobs = (int)order.size();
             // the blocks of the tables that are used: copy jobs, then block tables
@@ -723,7 +744,13 @@ namespace
                 else
                 {
                     int dm = (int)( g[0] / 16 ), fm = (int)( g[0] % 16 );
-                    if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.stream ), "batch launch" ) )
+                    if( mode == FN2_SERVE_PERSISTENT )
+                    {
+                        if( !fn2::HipOk( (hipError_t)fn2_launch_generate_queue( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.dCounter, s.stream ),
+                                         "work-queue launch" ) )
+                            return false;
+                    }
+                    else if( !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.stream ), "batch launch" ) )
                         return false;
                 }
                 first += (int)g[1];
@@ -844,7 +871,7 @@ FASTNOISE_API void* fn2ServeCreate( size_t hbmBudgetBytes, int maxBatchPoints, i
 {
     fn2::ClearError();
     if( !fn2::EnsureDevice() ) return nullptr;
-    if( mode < FN2_SERVE_PLAIN || mode > FN2_SERVE_GRAPH || maxBatchPoints <= 0 )
+    if( mode < FN2_SERVE_PLAIN || mode > FN2_SERVE_PERSISTENT || maxBatchPoints <= 0 )
     {
         fn2::SetError( "bad serving engine arguments" );
         return nullptr;

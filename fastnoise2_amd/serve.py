@@ -6,9 +6,9 @@ import weakref
 import numpy as np
 
 from . import _lib
-from ._lib import GenSpec, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, last_error
+from ._lib import GenSpec, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, SERVE_PERSISTENT, last_error
 
-__all__ = ["ServeEngine", "SERVE_PLAIN", "SERVE_BATCHED", "SERVE_GRAPH"]
+__all__ = ["ServeEngine", "SERVE_PLAIN", "SERVE_BATCHED", "SERVE_GRAPH", "SERVE_PERSISTENT"]
 
 
 class ServeEngine:

@@ -9,7 +9,7 @@ import pytest
 
 import fastnoise2_amd as fn
 from fastnoise2_amd import workloads as W
-from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH
+from fastnoise2_amd.serve import ServeEngine, SERVE_PLAIN, SERVE_BATCHED, SERVE_GRAPH, SERVE_PERSISTENT
 from fastnoise2_amd.tree import NodeSpec as S
 import oracle
 from conftest import assert_bitexact
@@ -57,7 +57,7 @@ def alone(nodes, t, spec):
     return gen(*list(spec.offset)[:d], *list(spec.count)[:d], *list(spec.step)[:d], seed=spec.seed)
 
 
-@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN])
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN, SERVE_PERSISTENT])
 def test_alone_equals_in_mix(gpu, mode):
     nodes = [fn.Node.from_encoded(t.encode()) for t in TREES]
     reqs = make_requests(120, seed=mode)
@@ -140,7 +140,7 @@ def test_admission_rejects_oversized_and_bad_requests(gpu):
         ServeEngine(hbm_budget_bytes=1 << 20, max_batch_points=1 << 22, mode=SERVE_BATCHED)
 
 
-@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN])
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN, SERVE_PERSISTENT])
 def test_outputs_to_pageable_pinned_and_device_memory(gpu, mode):
     # one batch holding all three destination kinds: staged through the slot's
     # pinned buffer, DMA straight into pinned memory, device-to-device
@@ -180,4 +180,26 @@ def test_graph_replays_with_varying_request_sizes(gpu):
             ref, ref_mm = node.gen_uniform_grid_2d(k * 1000.0, rep * 500.0, x, y, seed=k + rep)
             assert_bitexact(out, ref, "graph replay size %dx%d" % (x, y))
             assert mm == ref_mm
+    eng.close()
+
+
+def test_persistent_queue_with_more_tiles_than_resident_blocks(gpu):
+    # one batch of large requests of every tree: its tiles (256 points each at
+    # one point per lane) outnumber the resident blocks many times over, so
+    # every block pulls many tiles of different trees from the counter
+    nodes = [fn.Node.from_encoded(t.encode()) for t in TREES]
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 22, mode=SERVE_PERSISTENT)
+    tids = [eng.add_tree(t.encode()) for t in TREES]
+    reqs = []
+    for k in range(2):
+        for t in range(len(TREES)):
+            c = 96 if k == 0 else 57
+            reqs.append((t, fn.grid_spec(3, [k * 3000.0, -float(t) * 100, 7.0], [c, c, c], seed=11 * t + k)))
+    ids = [eng.submit(tids[t], spec) for t, spec in reqs]
+    for (t, spec), rid in zip(reqs, ids):
+        out, mm, _ = eng.wait(rid)
+        ref, ref_mm = alone(nodes, t, spec)
+        assert_bitexact(out, ref, "persistent tree %d count %d" % (t, spec.count[0]))
+        assert mm == ref_mm
+    assert eng.stats()["batches"] < len(reqs)
     eng.close()
