@@ -65,6 +65,9 @@ namespace fn2
         // pool carries the same padding)
         constexpr int kJitConstWindow = 16;
 
+        // Programs at least this long get register fences between instructions
+        constexpr int kJitFenceMinInstructions = 128;
+
         // The feature group an op needs compiled in (see fn2::kFeat*)
         int OpFeature( int op )
         {
@@ -109,8 +112,15 @@ namespace fn2
                  "static constexpr int kDims = " << p.dims << ";\n"
                  "static constexpr bool kBricks = " << ( PrunedCellular( p ) ? "true" : "false" ) << ";\n"
                  "__device__ __forceinline__ float operator()( const fn2::GenArgs& a, float* R ) const {\n";
+            // long programs get a register fence between instructions: measured
+            // on MI355X, a 355-instruction tree needs 256 VGPRs without (one wave
+            // per SIMD, 2.46 ms on 128^3) and 92 with (5 waves, 1.63 ms); a
+            // 145-instruction tree runs 10 % faster with them, while some
+            // 40-instruction trees lose 8 % to them
+            const bool fence = p.ins.size() >= (size_t)kJitFenceMinInstructions;
             for( const Ins& in : p.ins )
             {
+                if( fence && &in != &p.ins[0] ) s << "fn2k::registerFence<" << p.regs << ">( R );\n";
                 s << "{ constexpr fn2::Ins in = { " << (int)in.op << ", " << (int)in.dim << ", { ";
                 for( int i = 0; i < 4; i++ ) s << (int)in.p[i] << ( i < 3 ? ", " : " }, { " );
                 for( int i = 0; i < 6; i++ ) s << (int)in.r[i] << ( i < 5 ? ", " : " }, " );
@@ -227,7 +237,10 @@ namespace fn2
             }
 
             int mode = FN2_JIT_ASYNC;
-            size_t maxInstructions = 160;
+            // longer programs stay on the interpreter: compile time grows
+            // faster than linearly (about 15 s for 355 instructions on the
+            // MI355X host, most of it in LLVM's value-range propagation)
+            size_t maxInstructions = 512;
 
             const JitKernel* Get( const DeviceProgram* p )
             {

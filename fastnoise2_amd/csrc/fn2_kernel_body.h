@@ -50,6 +50,18 @@ namespace fn2k
         }
     }
 
+    // A compiler fence over the first N registers of a specialised program's
+    // register file: after it their values count as unknown, so the compiler
+    // cannot reuse a computation of one instruction in another (on long
+    // programs that kept hundreds of values live and left one wave per SIMD).
+    // It emits no instruction.
+    template<int N>
+    __device__ __forceinline__ void registerFence( float* R )
+    {
+#pragma unroll
+        for( int j = 0; j < N; j++ ) asm volatile( "" : "+v"( R[j] ) );
+    }
+
     // The register file of one point of a specialised program (every register
     // written before it is read, all indices compile-time constants): a fresh
     // private array per point, which lives in VGPRs and leaves the compiler no

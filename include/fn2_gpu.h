@@ -80,7 +80,7 @@ FASTNOISE_API int fn2SelfTestMath( int op, const float* dIn, float* dOut, int n 
  * compile in the background on first use, interpreter meanwhile. SYNC:
  * compile on first use. The FN2_JIT environment variable (off / async /
  * sync) sets the initial mode. fn2SetJitMode returns the previous mode.
- * Programs longer than FN2_JIT_MAX_INSTRUCTIONS (environment, default 160)
+ * Programs longer than FN2_JIT_MAX_INSTRUCTIONS (environment, default 512)
  * stay on the interpreter: their compile time grows to tens of seconds. */
 enum fn2JitMode
 {

@@ -71,6 +71,30 @@ def test_specialised_batches_in_serving(gpu, jit_sync):
     eng.close()
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_long_fenced_program_equals_interpreter_and_oracle(gpu, dims):
+    # the deepest mixed-serving tree (about 355 instructions): specialised
+    # with register fences between its instructions
+    enc = W.mixed_trees()[7]
+    node = fn.Node.from_encoded(enc)
+    assert node.program_info(dims)["instructions"] >= 128
+    off = [-101.0, 57.5, 13.0][:dims]
+    cnt = [48, 37, 11][:dims]
+    gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d}[dims]
+    old = fn.set_jit_mode(fn.JIT_OFF)
+    try:
+        interp, imm = gen(*off, *cnt, seed=-7)
+        fn.set_jit_mode(fn.JIT_SYNC)
+        spec_out, smm = gen(*off, *cnt, seed=-7)
+        assert node.jit_state(dims) == 1, fn.last_error()
+    finally:
+        fn.set_jit_mode(old)
+    ref, ref_mm = oracle.Tree(enc).grid(off, cnt, seed=-7)
+    assert_bitexact(interp, ref, "long tree interpreter")
+    assert_bitexact(spec_out, ref, "long tree specialised")
+    assert imm == smm == ref_mm
+
+
 # every fifth node case: all node families, dimensions mixed
 from cases import CASES, REGION  # noqa: E402
 

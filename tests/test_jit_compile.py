@@ -34,3 +34,17 @@ def test_specialised_source_compiles(name, spec, dims):
 def test_compile_check_rejects_null():
     assert fn.lib().fn2JitCompileCheck(None, 3, None) == 1
     assert "null" in fn.last_error()
+
+
+def test_long_programs_get_register_fences():
+    # programs of 128 instructions or more get a fence between instructions
+    # (one fewer than instructions); shorter ones none
+    long_tree = W.mixed_trees()[7]
+    node = fn.Node.from_encoded(long_tree)
+    n = node.program_info(3)["instructions"]
+    assert n >= 128
+    assert node.jit_source(3).count("registerFence<") == n - 1
+    for name, spec, dims in TREES:
+        short = fn.Node.from_encoded(spec.encode())
+        assert short.program_info(dims)["instructions"] < 128
+        assert "registerFence<" not in short.jit_source(dims)
