@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include "fn2_kernel_body.h"
 #include "fn2_kernel_args.h"
@@ -130,20 +131,23 @@ namespace
 {
     constexpr int kBlock = 256;
 
+    // Facts measured once per process; threads that race to measure one
+    // store the same value.
     int CUs()
     {
-        static int cus = 0;
-        if( cus == 0 )
+        static std::atomic<int> cus{ 0 };
+        int n = cus.load( std::memory_order_relaxed );
+        if( n == 0 )
         {
-            int dev = 0, n = 0;
+            int dev = 0;
             if( hipGetDevice( &dev ) != hipSuccess || hipDeviceGetAttribute( &n, hipDeviceAttributeMultiprocessorCount, dev ) != hipSuccess || n <= 0 )
             {
                 (void)hipGetLastError();
                 n = 256;
             }
-            cus = n;
+            cus.store( n, std::memory_order_relaxed );
         }
-        return cus;
+        return n;
     }
 
     // Blocks of a single-call (grid-stride) kernel. With the kernel's resident
@@ -177,6 +181,20 @@ namespace
             return 0;
         }
         return n;
+    }
+
+    // Resident blocks per CU of a kernel at a register-file size, from a
+    // per-kernel cache indexed by the size
+    template<class K>
+    int CachedResident( std::atomic<int>* cache, int regs, K kernel, size_t lds )
+    {
+        int r = cache[regs].load( std::memory_order_relaxed );
+        if( r == 0 )
+        {
+            r = std::max( 1, ResidentBlocks( kernel, lds ) );
+            cache[regs].store( r, std::memory_order_relaxed );
+        }
+        return r;
     }
 }
 
@@ -213,9 +231,9 @@ int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hi
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        static int resident[fn2::kMaxRegs + 1] = {};   /* per register count */ \
-        if( resident[args->regs] == 0 ) resident[args->regs] = std::max( 1, ResidentBlocks( fn2_generate<kBlock, DMV, FTV>, lds ) ); \
-        const unsigned grid = GridFor( args->total, resident[args->regs], 4 ); \
+        static std::atomic<int> resident[fn2::kMaxRegs + 1];   /* per register count, zero at start */ \
+        const int bpc = CachedResident( resident, args->regs, fn2_generate<kBlock, DMV, FTV>, lds ); \
+        const unsigned grid = GridFor( args->total, bpc, 4 ); \
         hipLaunchKernelGGL( ( fn2_generate<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), lds, stream, *args ); \
         return (int)hipGetLastError(); \
     }
@@ -255,9 +273,9 @@ int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJo
 #define FN2_TRY( DMV, FTV ) \
     if( ( dimMask & ~DMV ) == 0 && ( featMask & ~FTV ) == 0 ) \
     { \
-        static int resident[fn2::kMaxRegs + 1] = {};   /* per register count */ \
-        if( resident[regs] == 0 ) resident[regs] = std::max( 1, ResidentBlocks( fn2_generate_queue<kBlock, DMV, FTV>, lds ) ); \
-        const unsigned grid = (unsigned)std::min<int64_t>( nTiles, (int64_t)CUs() * resident[regs] ); \
+        static std::atomic<int> resident[fn2::kMaxRegs + 1];   /* per register count, zero at start */ \
+        const int bpc = CachedResident( resident, regs, fn2_generate_queue<kBlock, DMV, FTV>, lds ); \
+        const unsigned grid = (unsigned)std::min<int64_t>( nTiles, (int64_t)CUs() * bpc ); \
         hipLaunchKernelGGL( fn2_queue_reset, dim3( 1 ), dim3( 64 ), 0, stream, dCounter ); \
         hipError_t e = hipGetLastError(); \
         if( e != hipSuccess ) return (int)e; \
