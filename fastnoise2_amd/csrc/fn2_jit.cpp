@@ -142,7 +142,10 @@ namespace fn2
                  "{ fn2k::generateSingle<256>( a, Prog() ); }\n"
                  "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_batch( const fn2::GenArgs* __restrict__ jobs, "
                  "const int32_t* __restrict__ bj, const uint32_t* __restrict__ bb )\n"
-                 "{ fn2k::generateBatch<256>( jobs, bj, bb, Prog() ); }\n";
+                 "{ fn2k::generateBatch<256>( jobs, bj, bb, Prog() ); }\n"
+                 "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_queue( const fn2::GenArgs* __restrict__ jobs, "
+                 "const int32_t* __restrict__ tj, const uint32_t* __restrict__ tb, uint32_t nTiles, uint32_t* counter )\n"
+                 "{ fn2k::generateQueue<256>( jobs, tj, tb, nTiles, counter, Prog() ); }\n";
             return s.str();
         }
 
@@ -188,6 +191,7 @@ namespace fn2
             hipError_t e = hipModuleLoadData( &mod, code.data() );
             if( e == hipSuccess ) e = hipModuleGetFunction( &k.single, mod, "fn2_jit_single" );
             if( e == hipSuccess ) e = hipModuleGetFunction( &k.batch, mod, "fn2_jit_batch" );
+            if( e == hipSuccess ) e = hipModuleGetFunction( &k.queue, mod, "fn2_jit_queue" );
             if( e != hipSuccess )
             {
                 (void)hipGetLastError();
@@ -199,6 +203,11 @@ namespace fn2
             {
                 (void)hipGetLastError();
                 k.blocksPerCU = 0;
+            }
+            if( hipModuleOccupancyMaxActiveBlocksPerMultiprocessor( &k.queueBlocksPerCU, k.queue, 256, 0 ) != hipSuccess )
+            {
+                (void)hipGetLastError();
+                k.queueBlocksPerCU = 0;
             }
             return true;
         }

@@ -231,16 +231,37 @@ namespace fn2k
         generateTile<BLOCK>( jobs[blockJob[blockIdx.x]], blockBase[blockIdx.x], prog );
     }
 
+    // Publishes a wave's range of one job: a wave reduction, then global
+    // atomics of one wave-uniform value (the compiler issues them from one
+    // lane; it leaves generic-pointer atomics per lane). No-op without keys.
+    __device__ __forceinline__ void waveMinMax( float mn, float mx, int32_t* keys )
+    {
+        if( !keys ) return;
+#pragma unroll
+        for( int off = 32; off > 0; off >>= 1 )
+        {
+            mn = fminf( mn, __shfl_xor( mn, off ) );
+            mx = fmaxf( mx, __shfl_xor( mx, off ) );
+        }
+        // a wave without a number leaves the keys as they are
+        const int32_t kMin = mn <= mx ? orderedKey( mn ) : 0x7fffffff;
+        const int32_t kMax = mn <= mx ? orderedKey( mx ) : (int32_t)0x80000000u;
+        auto* g = (__attribute__( ( address_space( 1 ) ) ) int32_t*)keys;
+        __hip_atomic_fetch_min( g, __builtin_amdgcn_readfirstlane( kMin ), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+        __hip_atomic_fetch_max( g + 1, __builtin_amdgcn_readfirstlane( kMax ), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+    }
+
     // Work-queue form of a batch. The unit of work is a quarter of a tile:
     // item w covers the lanes (w % W) * 64 .. + 63 of tile w / W (W = waves
     // per block) exactly as that wave of block w / W runs it in
     // generateBatch. Every wave pulls items 0, 1, 2, ... from a device counter
-    // (zero at launch) until they pass W * nTiles. The loop has no barrier
-    // and no branch that only some lanes take around its back edge: all lanes
-    // of the wave take part in the pull (lane 0 adds one, the others zero)
-    // and in publishing the item's range (a wave reduction, then atomics of
-    // one wave-uniform value). The counter only grows, so every wave leaves
-    // after at most W * nTiles + 1 pulls and no wave waits for another.
+    // (zero at launch) until they pass W * nTiles, and keeps a running range
+    // of the job it is on, published when the job changes and at the end.
+    // The loop has no barrier and no branch that only some lanes take around
+    // its back edge: all lanes take part in the pull (lane 0 adds one, the
+    // others zero) and in publishing a range. The counter only grows, so
+    // every wave leaves after at most W * nTiles + 1 pulls and no wave waits
+    // for another.
     template<int BLOCK, class Prog>
     __device__ __forceinline__ void generateQueue( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ tileJob,
                                                    const uint32_t* __restrict__ tileBase, uint32_t nTiles, uint32_t* counter, Prog prog )
@@ -248,6 +269,8 @@ namespace fn2k
         constexpr uint32_t kWaves = BLOCK / 64;
         const uint32_t lane = threadIdx.x & 63;
         const uint32_t items = nTiles * kWaves;
+        int32_t* keys = nullptr;   // the job whose range mn, mx hold
+        float mn = __builtin_inff(), mx = -__builtin_inff();
         for( ;; )
         {
             const uint32_t pulled = atomicAdd( counter, lane == 0 ? 1u : 0u );
@@ -255,44 +278,53 @@ namespace fn2k
             if( w >= items ) break;
             const uint32_t t = w / kWaves;
             const fn2::GenArgs& a = jobs[tileJob[t]];
+            if( a.minmax != keys )
+            {
+                waveMinMax( mn, mx, keys );
+                keys = a.minmax;
+                mn = __builtin_inff();
+                mx = -__builtin_inff();
+            }
             const uint32_t total = (uint32_t)a.total;
             const uint32_t base = tileBase[t] + ( w % kWaves ) * 64 + lane;
             const int ppt = a.ppt;
-            const auto R = ldsRegs<BLOCK>( a.regs );   // each item starts from a cleared register file, as a block does
-            float mn = __builtin_inff(), mx = -__builtin_inff();
-#pragma unroll 1
-            for( int p = 0; p < ppt; p++ )
-            {
-                const uint32_t i = base + p * BLOCK;
-                if( i < total )
+            auto emit = [&]( uint32_t i, float v ) {
+                a.out[i] = v;
+                if( v == v )
                 {
-                    startPosition( a, i, R );
-                    const float v = prog( a, R );
-                    a.out[i] = v;
-                    if( v == v )
+                    mn = fminf( mn, v );
+                    mx = fmaxf( mx, v );
+                }
+            };
+            if constexpr( Prog::kFresh )
+            {
+#pragma unroll 1
+                for( int p = 0; p < ppt; p++ )
+                {
+                    const uint32_t i = base + p * BLOCK;
+                    if( i < total )
                     {
-                        mn = fminf( mn, v );
-                        mx = fmaxf( mx, v );
+                        RegFile fresh;
+                        startPositionD<Prog::kDims>( a, i, fresh.R );
+                        emit( i, prog( a, fresh.R ) );
                     }
                 }
             }
-            if( a.minmax )
+            else
             {
-#pragma unroll
-                for( int off = 32; off > 0; off >>= 1 )
+                const auto R = ldsRegs<BLOCK>( a.regs );   // each item starts from a cleared register file, as a block does
+#pragma unroll 1
+                for( int p = 0; p < ppt; p++ )
                 {
-                    mn = fminf( mn, __shfl_xor( mn, off ) );
-                    mx = fmaxf( mx, __shfl_xor( mx, off ) );
+                    const uint32_t i = base + p * BLOCK;
+                    if( i < total )
+                    {
+                        startPosition( a, i, R );
+                        emit( i, prog( a, R ) );
+                    }
                 }
-                // an item without a number leaves the keys as they are
-                const int32_t kMin = mn <= mx ? orderedKey( mn ) : 0x7fffffff;
-                const int32_t kMax = mn <= mx ? orderedKey( mx ) : (int32_t)0x80000000u;
-                // global-memory atomics of a uniform value: the compiler issues
-                // them from one lane (it leaves generic-pointer atomics per lane)
-                auto* keys = (__attribute__( ( address_space( 1 ) ) ) int32_t*)a.minmax;
-                __hip_atomic_fetch_min( keys, __builtin_amdgcn_readfirstlane( kMin ), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
-                __hip_atomic_fetch_max( keys + 1, __builtin_amdgcn_readfirstlane( kMax ), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
             }
         }
+        waveMinMax( mn, mx, keys );
     }
 }

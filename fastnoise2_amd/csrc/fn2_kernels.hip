@@ -53,17 +53,12 @@ __global__ void __launch_bounds__( BLOCK ) fn2_generate_batch( const fn2::GenArg
 
 // Persistent form for the serving layer: one launch of resident blocks per
 // batch pulls the batch's tiles (any tree, size and source) from a device
-// counter, which fn2_queue_reset zeroes on the same stream before the launch.
+// counter that is zero when the launch starts.
 template<int BLOCK, int DM, int FT>
 __global__ void __launch_bounds__( BLOCK ) fn2_generate_queue( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ tileJob,
                                                               const uint32_t* __restrict__ tileBase, uint32_t nTiles, uint32_t* counter )
 {
     fn2k::generateQueue<BLOCK>( jobs, tileJob, tileBase, nTiles, counter, Interpreted<DM, FT>() );
-}
-
-__global__ void fn2_queue_reset( uint32_t* counter )
-{
-    if( threadIdx.x == 0 ) *counter = 0;
 }
 
 __global__ void fn2_selftest_math( int op, const float* in, float* out, int n )
@@ -117,7 +112,8 @@ __global__ void __launch_bounds__( 256 ) fn2_bloxel_faces( const float* __restri
     }
 }
 
-__global__ void fn2_minmax_init( int32_t* keys, int n )
+// Resets n range-key pairs and zeroes nc work-queue counters
+__global__ void fn2_minmax_init( int32_t* keys, int n, uint32_t* counters, int nc )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if( i < n )
@@ -125,6 +121,7 @@ __global__ void fn2_minmax_init( int32_t* keys, int n )
         keys[2 * i] = 0x7fffffff;          // key of +NaN region: above every float
         keys[2 * i + 1] = (int32_t)0x80000000u;
     }
+    if( i < nc ) counters[i] = 0;
 }
 
 namespace
@@ -263,7 +260,8 @@ int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJ
 // The tiles of a batch through the work-queue kernel: a grid of at most the
 // resident blocks of the variant (every block is on a CU from the start and
 // pulls tiles until none is left). regs: the largest register count of the
-// jobs; counter: one device word, zeroed here on the stream first.
+// jobs; counter: one device word, zero when the launch starts (the serving
+// engine zeroes it with the batch's range keys, fn2_launch_batch_init).
 int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
                                int regs, uint32_t* dCounter, hipStream_t stream )
 {
@@ -276,9 +274,6 @@ int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJo
         static std::atomic<int> resident[fn2::kMaxRegs + 1];   /* per register count, zero at start */ \
         const int bpc = CachedResident( resident, regs, fn2_generate_queue<kBlock, DMV, FTV>, lds ); \
         const unsigned grid = (unsigned)std::min<int64_t>( nTiles, (int64_t)CUs() * bpc ); \
-        hipLaunchKernelGGL( fn2_queue_reset, dim3( 1 ), dim3( 64 ), 0, stream, dCounter ); \
-        hipError_t e = hipGetLastError(); \
-        if( e != hipSuccess ) return (int)e; \
         hipLaunchKernelGGL( ( fn2_generate_queue<kBlock, DMV, FTV> ), dim3( grid ), dim3( kBlock ), lds, stream, dJobs, dTileJob, dTileBase, \
                             (uint32_t)nTiles, dCounter ); \
         return (int)hipGetLastError(); \
@@ -309,7 +304,17 @@ int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t*
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream )
 {
     if( n <= 0 ) return 0;
-    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( n + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n );
+    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( n + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, (uint32_t*)nullptr, 0 );
+    return (int)hipGetLastError();
+}
+
+// The same, also zeroing nc work-queue counters (one launch per batch)
+int fn2_launch_batch_init( int32_t* keys, int n, uint32_t* counters, int nc, hipStream_t stream )
+{
+    if( n < 0 || nc < 0 || ( nc > 0 && !counters ) ) return (int)hipErrorInvalidValue;
+    const int m = std::max( n, nc );
+    if( m == 0 ) return 0;
+    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, counters, nc );
     return (int)hipGetLastError();
 }
 

@@ -23,9 +23,10 @@
 // batch's table upload, launches and copies into a HIP graph and replays the
 // graph when the next batch has the same shape (same job count, block count
 // and variant groups); job records and tables change, the topology does not.
-// FN2_SERVE_PERSISTENT runs each batch, whatever its trees, as one launch of
-// the interpreter's work-queue kernel: a grid of resident blocks pulls the
-// batch's tiles from a device counter (specialised kernels are not used).
+// FN2_SERVE_PERSISTENT runs each batch through work-queue kernels: a grid of
+// resident blocks pulls tiles from a device counter, one launch per
+// specialised kernel in the batch and one interpreter launch for all other
+// trees together.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <array>
@@ -53,6 +54,7 @@ int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJ
 int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
 int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
                                int regs, uint32_t* dCounter, hipStream_t stream );
+int fn2_launch_batch_init( int32_t* keys, int n, uint32_t* counters, int nc, hipStream_t stream );
 int fn2_batch_block_threads( void );
 }
 
@@ -108,7 +110,7 @@ namespace
         hipEvent_t done = nullptr;
         float* dOut = nullptr;          // capacity floats
         float* hOut = nullptr;          // pinned
-        int32_t* dKeys = nullptr;       // 2 per job, then the tile counter of the persistent mode
+        int32_t* dKeys = nullptr;       // 2 per job, then the persistent mode's tile counters (1 per launch)
         uint32_t* dCounter = nullptr;
         int32_t* hKeys = nullptr;       // pinned
         GenArgs* dJobs = nullptr;
@@ -238,7 +240,7 @@ namespace
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
-            const size_t keyBytes = 2 * sizeof( int32_t ) * kMaxJobs + 256;
+            const size_t keyBytes = 3 * sizeof( int32_t ) * kMaxJobs;   // a launch per job at most
             size_t perSlot = (size_t)capacity * 4 + jobsBytes + bjBytes + bbBytes + keyBytes;
             int nSlots = mode == FN2_SERVE_PLAIN ? 1 : kSlots;
             if( perSlot * nSlots > budget )
@@ -270,7 +272,10 @@ namespace
             {
                 int dev = 0, cus = 0;
                 if( hipGetDevice( &dev ) == hipSuccess && hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) == hipSuccess && cus > 0 )
+                {
                     fillBlocks = 2 * cus;
+                    this->cus = cus;
+                }
                 (void)hipGetLastError();
             }
             if( mode != FN2_SERVE_PLAIN )
@@ -585,6 +590,7 @@ namespace
         int64_t deviceBytes = 0;
         int maxBlocks = 0;
         int fillBlocks = 512;   // blocks that fill the GPU twice over (2 per CU)
+        int cus = 256;          // compute units of the device
         int nextSlot = 0;
         std::thread dispatcher, completer;
         std::unique_ptr<CopyPool> copies;
@@ -620,7 +626,7 @@ namespace
         {
             // jobs sorted by kernel so that each kernel's blocks are contiguous
             std::vector<Request*> order( s.reqs );
-            for( Request* r : order ) r->jit = mode == FN2_SERVE_PERSISTENT ? nullptr : fn2::GetJitKernel( r->prog );
+            for( Request* r : order ) r->jit = fn2::GetJitKernel( r->prog );
             std::stable_sort( order.begin(), order.end(),
                               []( Request* a, Request* b ) { return KernelKey( a->prog, a->jit ) < KernelKey( b->prog, b->jit ); } );
             GenArgs* hJobs = (GenArgs*)s.hTables;
@@ -669,18 +675,25 @@ namespace
                 }
                 nb += rb;
             }
-            if( mode == FN2_SERVE_PERSISTENT && shape.size() > 1 )
+            if( mode == FN2_SERVE_PERSISTENT )
             {
-                // one work-queue launch: the interpreter variant covering every job
-                int64_t dm = 0, fm = 0, regs = 0;
-                for( const auto& g : shape )
+                // the interpreter jobs (sorted first: their keys are below 256)
+                // in one work-queue launch of the variant covering them all
+                size_t ni = 0;
+                int64_t dm = 0, fm = 0, regs = 0, blocks = 0;
+                for( ; ni < shape.size() && !kernels[ni]; ni++ )
                 {
-                    dm |= g[0] / 16;
-                    fm |= g[0] % 16;
-                    regs = std::max( regs, g[2] );
+                    dm |= shape[ni][0] / 16;
+                    fm |= shape[ni][0] % 16;
+                    regs = std::max( regs, shape[ni][2] );
+                    blocks += shape[ni][1];
                 }
-                shape.assign( 1, { dm * 16 + fm, nb, regs } );
-                kernels.assign( 1, nullptr );
+                if( ni > 1 )
+                {
+                    shape.erase( shape.begin() + 1, shape.begin() + ni );
+                    kernels.erase( kernels.begin() + 1, kernels.begin() + ni );
+                    shape[0] = { dm * 16 + fm, blocks, regs };
+                }
             }
             const int nJobs = (int)order.size();
             // the blocks of the tables that are used: copy jobs, then block tables
@@ -726,14 +739,32 @@ namespace
             if( !fn2::HipOk( hipMemcpyAsync( s.dJobs, s.hTables, usedJob, hipMemcpyHostToDevice, s.stream ), "upload jobs" ) ) return false;
             if( !fn2::HipOk( hipMemcpyAsync( s.dBlockJob, s.hTables + jobsBytes, sizeof( int32_t ) * nb, hipMemcpyHostToDevice, s.stream ), "upload blocks" ) ) return false;
             if( !fn2::HipOk( hipMemcpyAsync( s.dBlockBase, s.hTables + jobsBytes + bjBytes, sizeof( uint32_t ) * nb, hipMemcpyHostToDevice, s.stream ), "upload blocks" ) ) return false;
-            if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, s.stream ), "range reset" ) ) return false;
+            if( mode == FN2_SERVE_PERSISTENT )
+            {
+                // range keys and one zero counter per work-queue launch, in one kernel
+                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, s.dCounter, (int)shape.size(), s.stream ), "range reset" ) )
+                    return false;
+            }
+            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, s.stream ), "range reset" ) ) return false;
             int first = 0;
             for( size_t gi = 0; gi < shape.size(); gi++ )
             {
                 const auto& g = shape[gi];
                 const int32_t* bj = s.dBlockJob + first;
                 const uint32_t* bb = s.dBlockBase + first;
-                if( kernels[gi] )
+                if( kernels[gi] && mode == FN2_SERVE_PERSISTENT )
+                {
+                    const fn2::GenArgs* jobs = s.dJobs;
+                    uint32_t nTiles = (uint32_t)g[1];
+                    uint32_t* counter = s.dCounter + gi;
+                    const int64_t resident = (int64_t)cus * std::max( 1, kernels[gi]->queueBlocksPerCU );
+                    const unsigned grid = (unsigned)std::min<int64_t>( g[1], resident );
+                    void* params[] = { (void*)&jobs, (void*)&bj, (void*)&bb, (void*)&nTiles, (void*)&counter };
+                    if( !fn2::HipOk( hipModuleLaunchKernel( kernels[gi]->queue, grid, 1, 1, kThreads, 1, 1, 0, s.stream, params, nullptr ),
+                                     "specialised work-queue launch" ) )
+                        return false;
+                }
+                else if( kernels[gi] )
                 {
                     const fn2::GenArgs* jobs = s.dJobs;
                     void* params[] = { (void*)&jobs, (void*)&bj, (void*)&bb };
@@ -746,7 +777,7 @@ namespace
                     int dm = (int)( g[0] / 16 ), fm = (int)( g[0] % 16 );
                     if( mode == FN2_SERVE_PERSISTENT )
                     {
-                        if( !fn2::HipOk( (hipError_t)fn2_launch_generate_queue( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.dCounter, s.stream ),
+                        if( !fn2::HipOk( (hipError_t)fn2_launch_generate_queue( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.dCounter + gi, s.stream ),
                                          "work-queue launch" ) )
                             return false;
                     }

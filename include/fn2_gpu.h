@@ -118,7 +118,7 @@ enum fn2ServeMode
     FN2_SERVE_PLAIN = 0,    /* one launch + copy per request (the per-chunk baseline) */
     FN2_SERVE_BATCHED = 1,  /* requests packed into one launch per batch, overlapped with host work */
     FN2_SERVE_GRAPH = 2,    /* batched, each batch replayed through a captured hipGraph */
-    FN2_SERVE_PERSISTENT = 3, /* each batch as one launch of resident blocks pulling tiles of any tree from a device counter */
+    FN2_SERVE_PERSISTENT = 3, /* each batch through work-queue kernels: resident blocks pull tiles from a device counter */
 };
 
 /* Creates an engine. hbmBudgetBytes bounds the device bytes held by queued and

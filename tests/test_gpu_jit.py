@@ -6,7 +6,7 @@ import pytest
 
 import fastnoise2_amd as fn
 from fastnoise2_amd import workloads as W
-from fastnoise2_amd.serve import ServeEngine, SERVE_BATCHED
+from fastnoise2_amd.serve import ServeEngine, SERVE_BATCHED, SERVE_PERSISTENT
 from fastnoise2_amd.tree import NodeSpec as S
 import oracle
 from conftest import assert_bitexact
@@ -53,14 +53,18 @@ def test_specialised_equals_interpreter_and_oracle(gpu, name, spec, dims):
     assert imm == smm == ref_mm
 
 
-def test_specialised_batches_in_serving(gpu, jit_sync):
-    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=SERVE_BATCHED)
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_PERSISTENT])
+def test_specialised_batches_in_serving(gpu, jit_sync, mode):
+    # batched: the specialised batch kernels; persistent: their work-queue
+    # entry points (tiles pulled from a counter), one launch per tree
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=mode)
     tids = [eng.add_tree(t[1].encode()) for t in TREES[:3]]
     reqs = []
     for k in range(18):
         t = k % 3
         d = TREES[t][2]
-        reqs.append((t, fn.grid_spec(d, [k * 17.0] * d, [24] * d, seed=k)))
+        c = 24 if k < 15 else 48      # the last ones have hundreds of tiles
+        reqs.append((t, fn.grid_spec(d, [k * 17.0] * d, [c] * d, seed=k)))
     ids = [eng.submit(tids[t], spec) for t, spec in reqs]
     for (t, spec), rid in zip(reqs, ids):
         out, mm, _ = eng.wait(rid)
