@@ -34,8 +34,8 @@ namespace fn2
         size_t outCap = 0;
         float* dPos = nullptr;
         size_t posCap = 0;
-        int32_t* dKeys = nullptr;
-        int32_t* hKeys = nullptr;   // pinned
+        int32_t* dKeys = nullptr;   // one range record, slots kRangeStrideSpread ints apart
+        int32_t* hKeys = nullptr;   // pinned, the same size
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
         bool Reserve( float*& buf, size_t& cap, size_t floats );
@@ -58,6 +58,11 @@ namespace fn2
 
     FastDiv MakeFastDiv( uint32_t d );
     float DecodeKey( int32_t k );
+    // The range a record holds: false when no number was published to it
+    // (every output NaN, or none), else its min and max
+    bool DecodeRange( const int32_t* record, int stride, float& min, float& max );
+    // ints of a single call's range record
+    constexpr int kCallRangeInts = kRangeSlots * kRangeStrideSpread;
     bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err );
 
     const DeviceProgram* GetDeviceProgram( const Program& p );

@@ -7,6 +7,19 @@
 
 namespace fn2
 {
+    // A range record: the output range of one call as ordered-int keys, kept
+    // in kRangeSlots {min, max} pairs `stride` ints apart. A block (or a
+    // wave) publishes to the slot of its index, so that its atomics do not
+    // queue behind every other block's on one address; the range is the
+    // smallest min key and the largest max key over the slots. Measured on
+    // MI355X (1 slot -> 16 compact slots -> 16 slots on their own 128-byte
+    // lines): Perlin 64^3 15.0 -> 9.1 -> 4.4 us, a 1 M-point 4D call 56 -> 41
+    // -> 40 us; single calls take the spread layout, the serving engine
+    // (which reads a record back per request) the compact one.
+    constexpr int kRangeSlots = 16;
+    constexpr int kRangeStrideSpread = 32;    // a slot per 128-byte line
+    constexpr int kRangeStrideCompact = 2;    // slots side by side
+
     struct GenArgs
     {
         const Ins* ins;          // device program
@@ -18,12 +31,13 @@ namespace fn2
         int32_t seed;
         int32_t total;           // points (the reference counts in int)
         float* out;              // device output, `total` floats
-        int32_t* minmax;         // device ordered-int keys {min, max}, or null
+        int32_t* minmax;         // device range record (kRangeInts ordered-int keys), or null
         FastDiv div[3];          // grid: divisors by x, y, z count; tileable: div[0] = x size
         float offset[4];
         float step[4];
         const float* pos[4];     // position arrays (device); tileable: cos x, cos y, sin x, sin y tables
         int32_t ppt;             // batched kernel: points per lane of each block of this job
         int32_t regs;            // registers the program uses (the interpreter's file holds as many)
+        int32_t rangeStride;     // ints between the slots of the range record
     };
 }

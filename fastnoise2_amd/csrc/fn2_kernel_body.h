@@ -18,8 +18,9 @@ namespace fn2k
     using namespace fn2i;
 
     template<int BLOCK>
-    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* keys )
+    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* record, int stride )
     {
+        int32_t* keys = record + ( blockIdx.x % fn2::kRangeSlots ) * stride;
         __shared__ float sMin[BLOCK / 64], sMax[BLOCK / 64];
 #pragma unroll
         for( int off = 32; off > 0; off >>= 1 )
@@ -169,7 +170,7 @@ namespace fn2k
                 emit( i, prog( a, R ) );
             }
         }
-        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride );
     }
 
     // One tile of a batched request: points [first, first + BLOCK * ppt) of
@@ -218,7 +219,7 @@ namespace fn2k
                 }
             }
         }
-        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax );
+        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride );
     }
 
     // Batched requests: block b runs tile (blockJob[b], blockBase[b]).
@@ -231,12 +232,14 @@ namespace fn2k
         generateTile<BLOCK>( jobs[blockJob[blockIdx.x]], blockBase[blockIdx.x], prog );
     }
 
-    // Publishes a wave's range of one job: a wave reduction, then global
-    // atomics of one wave-uniform value (the compiler issues them from one
-    // lane; it leaves generic-pointer atomics per lane). No-op without keys.
-    __device__ __forceinline__ void waveMinMax( float mn, float mx, int32_t* keys )
+    // Publishes a wave's range of one job to slot `slot` of its range record:
+    // a wave reduction, then global atomics of one wave-uniform value (the
+    // compiler issues them from one lane; it leaves generic-pointer atomics
+    // per lane). No-op without a record.
+    __device__ __forceinline__ void waveMinMax( float mn, float mx, int32_t* record, int slot, int stride )
     {
-        if( !keys ) return;
+        if( !record ) return;
+        int32_t* keys = record + slot * stride;
 #pragma unroll
         for( int off = 32; off > 0; off >>= 1 )
         {
@@ -269,8 +272,10 @@ namespace fn2k
         constexpr uint32_t kWaves = BLOCK / 64;
         const uint32_t lane = threadIdx.x & 63;
         const uint32_t items = nTiles * kWaves;
-        int32_t* keys = nullptr;   // the job whose range mn, mx hold
+        int32_t* keys = nullptr;   // the record of the job whose range mn, mx hold
+        int keyStride = 0;
         float mn = __builtin_inff(), mx = -__builtin_inff();
+        const int slot = __builtin_amdgcn_readfirstlane( (int)( ( blockIdx.x * kWaves + ( threadIdx.x >> 6 ) ) % fn2::kRangeSlots ) );
         for( ;; )
         {
             const uint32_t pulled = atomicAdd( counter, lane == 0 ? 1u : 0u );
@@ -280,8 +285,9 @@ namespace fn2k
             const fn2::GenArgs& a = jobs[tileJob[t]];
             if( a.minmax != keys )
             {
-                waveMinMax( mn, mx, keys );
+                waveMinMax( mn, mx, keys, slot, keyStride );
                 keys = a.minmax;
+                keyStride = a.rangeStride;
                 mn = __builtin_inff();
                 mx = -__builtin_inff();
             }
@@ -325,6 +331,6 @@ namespace fn2k
                 }
             }
         }
-        waveMinMax( mn, mx, keys );
+        waveMinMax( mn, mx, keys, slot, keyStride );
     }
 }

@@ -8,11 +8,13 @@
 // GPR indexing, no scratch) and writes one float with a coalesced store.
 //
 // Output range: every lane keeps a running min/max in registers; a block
-// reduces them through wave shuffles and LDS and publishes one atomic pair.
-// Same-address atomics serialise in L2, so the number of blocks, not of
-// waves, bounds their cost: the single-call kernel runs a grid-stride loop
-// over a grid sized to fill the GPU (a few thousand blocks), and the batched
-// kernel gives each block a tile of several points per lane.
+// reduces them through wave shuffles and LDS and publishes one atomic pair
+// to the slot of its index in the call's range record (fn2_kernel_args.h).
+// Same-address atomics queue one behind the other, so the slots spread them;
+// the number of blocks still bounds their cost: the single-call kernel runs a
+// grid-stride loop over a grid sized to fill the GPU (a few thousand
+// blocks), and the batched kernel gives each block a tile of several points
+// per lane.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
@@ -112,14 +114,16 @@ __global__ void __launch_bounds__( 256 ) fn2_bloxel_faces( const float* __restri
     }
 }
 
-// Resets n range-key pairs and zeroes nc work-queue counters
-__global__ void fn2_minmax_init( int32_t* keys, int n, uint32_t* counters, int nc )
+// Resets n consecutive range records (every slot, `stride` ints apart) and
+// zeroes nc work-queue counters
+__global__ void fn2_minmax_init( int32_t* keys, int n, int stride, uint32_t* counters, int nc )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if( i < n )
+    if( i < n * fn2::kRangeSlots )
     {
-        keys[2 * i] = 0x7fffffff;          // key of +NaN region: above every float
-        keys[2 * i + 1] = (int32_t)0x80000000u;
+        int32_t* k = keys + i * stride;   // slot i % kRangeSlots of record i / kRangeSlots
+        k[0] = 0x7fffffff;          // key of +NaN region: above every float
+        k[1] = (int32_t)0x80000000u;
     }
     if( i < nc ) counters[i] = 0;
 }
@@ -301,20 +305,22 @@ int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t*
     return (int)hipGetLastError();
 }
 
-int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream )
+int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream )
 {
     if( n <= 0 ) return 0;
-    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( n + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, (uint32_t*)nullptr, 0 );
+    if( stride < 2 ) return (int)hipErrorInvalidValue;
+    const int m = n * fn2::kRangeSlots;
+    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, stride, (uint32_t*)nullptr, 0 );
     return (int)hipGetLastError();
 }
 
 // The same, also zeroing nc work-queue counters (one launch per batch)
-int fn2_launch_batch_init( int32_t* keys, int n, uint32_t* counters, int nc, hipStream_t stream )
+int fn2_launch_batch_init( int32_t* keys, int n, int stride, uint32_t* counters, int nc, hipStream_t stream )
 {
-    if( n < 0 || nc < 0 || ( nc > 0 && !counters ) ) return (int)hipErrorInvalidValue;
-    const int m = std::max( n, nc );
+    if( n < 0 || nc < 0 || stride < 2 || ( nc > 0 && !counters ) ) return (int)hipErrorInvalidValue;
+    const int m = std::max( n * fn2::kRangeSlots, nc );
     if( m == 0 ) return 0;
-    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, counters, nc );
+    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, stride, counters, nc );
     return (int)hipGetLastError();
 }
 

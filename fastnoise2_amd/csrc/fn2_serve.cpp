@@ -51,10 +51,10 @@ extern "C" {
 int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
 int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask,
                                int regs, hipStream_t stream );
-int fn2_launch_minmax_init( int32_t* keys, int n, hipStream_t stream );
+int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream );
 int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
                                int regs, uint32_t* dCounter, hipStream_t stream );
-int fn2_launch_batch_init( int32_t* keys, int n, uint32_t* counters, int nc, hipStream_t stream );
+int fn2_launch_batch_init( int32_t* keys, int n, int stride, uint32_t* counters, int nc, hipStream_t stream );
 int fn2_batch_block_threads( void );
 }
 
@@ -65,6 +65,8 @@ namespace
 
     // threads per block of the batched kernel; requests start on multiples of it
     const int kThreads = fn2_batch_block_threads();
+    // a request's range record: compact (the engine reads one back per request)
+    constexpr int kJobRangeInts = fn2::kRangeSlots * fn2::kRangeStrideCompact;
     // candidate points per lane of the tuner, and its calibration size
     constexpr int kPptCandidates[] = { 1, 2, 4, 8 };
     constexpr int kTunePoints = 1 << 20;
@@ -110,7 +112,7 @@ namespace
         hipEvent_t done = nullptr;
         float* dOut = nullptr;          // capacity floats
         float* hOut = nullptr;          // pinned
-        int32_t* dKeys = nullptr;       // 2 per job, then the persistent mode's tile counters (1 per launch)
+        int32_t* dKeys = nullptr;       // a range record per job, then the persistent mode's tile counters (1 per launch)
         uint32_t* dCounter = nullptr;
         int32_t* hKeys = nullptr;       // pinned
         GenArgs* dJobs = nullptr;
@@ -240,7 +242,7 @@ namespace
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
-            const size_t keyBytes = 3 * sizeof( int32_t ) * kMaxJobs;   // a launch per job at most
+            const size_t keyBytes = ( kJobRangeInts + 1 ) * sizeof( int32_t ) * kMaxJobs;   // a launch per job at most
             size_t perSlot = (size_t)capacity * 4 + jobsBytes + bjBytes + bbBytes + keyBytes;
             int nSlots = mode == FN2_SERVE_PLAIN ? 1 : kSlots;
             if( perSlot * nSlots > budget )
@@ -257,14 +259,14 @@ namespace
                     !fn2::HipOk( hipMalloc( (void**)&s.dOut, (size_t)capacity * 4 ), "serve arena" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hOut, (size_t)capacity * 4, hipHostMallocDefault ), "serve staging" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dKeys, keyBytes ), "serve keys" ) ||
-                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, 2 * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
+                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, kJobRangeInts * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dJobs, s.tableBytes ), "serve tables" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hTables, s.tableBytes, hipHostMallocDefault ), "serve tables" ) )
                 {
                     initError = fn2::LastError();
                     return false;
                 }
-                s.dCounter = (uint32_t*)( s.dKeys + 2 * kMaxJobs );
+                s.dCounter = (uint32_t*)( s.dKeys + (size_t)kJobRangeInts * kMaxJobs );
                 s.dBlockJob = (int32_t*)( (uint8_t*)s.dJobs + jobsBytes );
                 s.dBlockBase = (uint32_t*)( (uint8_t*)s.dJobs + jobsBytes + bjBytes );
             }
@@ -285,13 +287,13 @@ namespace
                     !fn2::HipOk( hipEventCreate( &tuneEv0 ), "tune event" ) || !fn2::HipOk( hipEventCreate( &tuneEv1 ), "tune event" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneOut, (size_t)kTunePoints * 4 ), "tune arena" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneJob, sizeof( GenArgs ) ), "tune job" ) ||
-                    !fn2::HipOk( hipMalloc( (void**)&tuneKeys, 2 * sizeof( int32_t ) ), "tune keys" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&tuneKeys, kJobRangeInts * sizeof( int32_t ) ), "tune keys" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneBJ, tb * sizeof( int32_t ) * 2 ), "tune tables" ) )
                 {
                     initError = fn2::LastError();
                     return false;
                 }
-                deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + 8 + tb * 8 );
+                deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + kJobRangeInts * 4 + tb * 8 );
             }
             copies.reset( new CopyPool( 3 ) );
             dispatcher = std::thread( [this] { DispatchLoop(); } );
@@ -526,6 +528,7 @@ namespace
             a.resultReg = prog->resultReg;
             a.out = tuneOut;
             a.minmax = tuneKeys;
+            a.rangeStride = fn2::kRangeStrideCompact;
             int best = 1;
             float bestMs = 1e30f;
             std::vector<int32_t> bj;
@@ -548,7 +551,7 @@ namespace
                 float ms = 1e30f;
                 for( int rep = 0; rep < 3; rep++ )
                 {
-                    if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, tuneStream ), "tune launch" ) ||
+                    if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, fn2::kRangeStrideCompact, tuneStream ), "tune launch" ) ||
                         !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
                         !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ), "tune launch" ) ||
                         !fn2::HipOk( hipEventRecord( tuneEv1, tuneStream ), "tune event" ) ||
@@ -652,7 +655,8 @@ namespace
                 a.out = s.dOut + r->outOffset;
                 const int ppt = std::min( r->ppt, pptCap );
                 a.ppt = ppt;
-                a.minmax = s.dKeys + 2 * j;
+                a.minmax = s.dKeys + (size_t)kJobRangeInts * j;
+                a.rangeStride = fn2::kRangeStrideCompact;
                 hJobs[j] = a;
                 const int64_t tile = (int64_t)kThreads * ppt;
                 int rb = (int)( ( r->points + tile - 1 ) / tile );
@@ -742,10 +746,10 @@ namespace
             if( mode == FN2_SERVE_PERSISTENT )
             {
                 // range keys and one zero counter per work-queue launch, in one kernel
-                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, s.dCounter, (int)shape.size(), s.stream ), "range reset" ) )
+                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, fn2::kRangeStrideCompact, s.dCounter, (int)shape.size(), s.stream ), "range reset" ) )
                     return false;
             }
-            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, s.stream ), "range reset" ) ) return false;
+            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, fn2::kRangeStrideCompact, s.stream ), "range reset" ) ) return false;
             int first = 0;
             for( size_t gi = 0; gi < shape.size(); gi++ )
             {
@@ -804,7 +808,7 @@ namespace
                     return false;
             }
             if( stagedEnd && !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, stagedEnd * 4, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
-            if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, 2 * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
+            if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, kJobRangeInts * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
             return true;
         }
 
@@ -862,16 +866,10 @@ namespace
                     copies->Run( segs );
                     for( Request* r : s.reqs )
                     {
-                        int32_t k0 = s.hKeys[2 * r->job], k1 = s.hKeys[2 * r->job + 1];
-                        if( k0 == 0x7fffffff && k1 == (int32_t)0x80000000u )
+                        if( !fn2::DecodeRange( s.hKeys + (size_t)r->job * kJobRangeInts, fn2::kRangeStrideCompact, r->mm[0], r->mm[1] ) )
                         {
                             r->mm[0] = __builtin_inff();
                             r->mm[1] = -__builtin_inff();
-                        }
-                        else
-                        {
-                            r->mm[0] = fn2::DecodeKey( k0 );
-                            r->mm[1] = fn2::DecodeKey( k1 );
                         }
                     }
                 }
