@@ -300,7 +300,10 @@ int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t*
     // inside the gen^3 block for 3 <= gen <= 1290
     if( !density || !counts || gen < 3 || gen > 1290 ) return (int)hipErrorInvalidValue;
     const int64_t s = gen - 2, total = s * s * s;
-    const unsigned blocks = std::max( 1u, GridFor( total ) );
+    // at most a block per CU: every block adds into the same two counters,
+    // and those atomics queue one behind the other (a 64^3 chunk would
+    // otherwise take 1024 blocks)
+    const unsigned blocks = std::max( 1u, std::min<unsigned>( GridFor( total ), (unsigned)CUs() ) );
     hipLaunchKernelGGL( fn2_bloxel_faces, dim3( blocks ), dim3( 256 ), 0, stream, density, gen, iso, counts );
     return (int)hipGetLastError();
 }
