@@ -152,13 +152,12 @@ namespace
     }
 
     // Blocks of a single-call (grid-stride) kernel. With the kernel's resident
-    // blocks per CU known: `rounds` times what the GPU holds at once (measured
-    // on MI355X: 2 for specialised kernels, 4 for the interpreter; more blocks
-    // cost more range atomics), but never fewer than 2 points per thread, since
-    // the per-thread prologue and the block's range reduction cost about as much
-    // as a point (a 1 M-point 4D call runs 1.8x faster at 2 points per thread
-    // than at 1). Without it, 16 blocks per CU.
-    unsigned GridFor( int64_t total, int blocksPerCU = 0, int rounds = 2 )
+    // blocks per CU known: 4 rounds of what the GPU holds at once, and at
+    // least 2 points per thread (measured on MI355X with the range record's
+    // spread slots: 4 rounds 2-5 % faster than 2 on the cellular and serving
+    // trees, the same on FBm; 1 or 2 points per thread the same). Without
+    // it, 16 blocks per CU.
+    unsigned GridFor( int64_t total, int blocksPerCU = 0, int rounds = 4 )
     {
         if( blocksPerCU <= 0 )
         {
