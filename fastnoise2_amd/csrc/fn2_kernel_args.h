@@ -20,6 +20,13 @@ namespace fn2
     constexpr int kRangeStrideSpread = 32;    // a slot per 128-byte line
     constexpr int kRangeStrideCompact = 2;    // slots side by side
 
+    // A work-queue launch's counters: kQueueCounters of them, each on its own
+    // 128-byte line (kQueueCounterStride uint32 apart); counter c hands out
+    // the c-th sixteenth of the items beyond the first one per wave.
+    constexpr int kQueueCounters = 16;
+    constexpr int kQueueCounterStride = 32;
+    constexpr int kQueueCounterInts = kQueueCounters * kQueueCounterStride;
+
     struct GenArgs
     {
         const Ins* ins;          // device program

@@ -263,8 +263,9 @@ int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJ
 // The tiles of a batch through the work-queue kernel: a grid of at most the
 // resident blocks of the variant (every block is on a CU from the start and
 // pulls tiles until none is left). regs: the largest register count of the
-// jobs; counter: one device word, zero when the launch starts (the serving
-// engine zeroes it with the batch's range keys, fn2_launch_batch_init).
+// jobs; counter: kQueueCounterInts device words, zero when the launch starts
+// (the serving engine zeroes them with the batch's range keys,
+// fn2_launch_batch_init).
 int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
                                int regs, uint32_t* dCounter, hipStream_t stream )
 {

@@ -65,8 +65,8 @@ namespace
 
     // threads per block of the batched kernel; requests start on multiples of it
     const int kThreads = fn2_batch_block_threads();
-    // a request's range record: compact (the engine reads one back per request)
-    constexpr int kJobRangeInts = fn2::kRangeSlots * fn2::kRangeStrideCompact;
+    // the tuner's range record: compact
+    constexpr int kTuneRangeInts = fn2::kRangeSlots * fn2::kRangeStrideCompact;
     // candidate points per lane of the tuner, and its calibration size
     constexpr int kPptCandidates[] = { 1, 2, 4, 8 };
     constexpr int kTunePoints = 1 << 20;
@@ -112,7 +112,7 @@ namespace
         hipEvent_t done = nullptr;
         float* dOut = nullptr;          // capacity floats
         float* hOut = nullptr;          // pinned
-        int32_t* dKeys = nullptr;       // a range record per job, then the persistent mode's tile counters (1 per launch)
+        int32_t* dKeys = nullptr;       // a range record per job, then the persistent mode's work counters (kQueueCounterInts per launch)
         uint32_t* dCounter = nullptr;
         int32_t* hKeys = nullptr;       // pinned
         GenArgs* dJobs = nullptr;
@@ -226,12 +226,20 @@ namespace
     {
     public:
         int mode;
+        int jobStride = fn2::kRangeStrideCompact;   // ints between a request's range slots
+        int jobRangeInts = fn2::kRangeSlots * fn2::kRangeStrideCompact;
         int64_t capacity;       // points per batch
         size_t budget;
         std::string initError;
 
         Engine( size_t hbmBudget, int maxBatchPoints, int m ) : mode( m ), capacity( maxBatchPoints ), budget( hbmBudget )
         {
+            // A request's range record: the engine reads one back per
+            // request, so compact; the work-queue kernels publish a range per
+            // wave (four times as many atomics as a block-per-tile launch),
+            // which queue up within one line, so they get a line per slot
+            jobStride = mode == FN2_SERVE_PERSISTENT ? fn2::kRangeStrideSpread : fn2::kRangeStrideCompact;
+            jobRangeInts = fn2::kRangeSlots * jobStride;
         }
 
         bool Init()
@@ -242,7 +250,8 @@ namespace
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
-            const size_t keyBytes = ( kJobRangeInts + 1 ) * sizeof( int32_t ) * kMaxJobs;   // a launch per job at most
+            // range records, then (persistent mode) the counters of up to a launch per job
+            const size_t keyBytes = ( jobRangeInts + ( mode == FN2_SERVE_PERSISTENT ? fn2::kQueueCounterInts : 0 ) ) * sizeof( int32_t ) * kMaxJobs;
             size_t perSlot = (size_t)capacity * 4 + jobsBytes + bjBytes + bbBytes + keyBytes;
             int nSlots = mode == FN2_SERVE_PLAIN ? 1 : kSlots;
             if( perSlot * nSlots > budget )
@@ -259,14 +268,14 @@ namespace
                     !fn2::HipOk( hipMalloc( (void**)&s.dOut, (size_t)capacity * 4 ), "serve arena" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hOut, (size_t)capacity * 4, hipHostMallocDefault ), "serve staging" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dKeys, keyBytes ), "serve keys" ) ||
-                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, kJobRangeInts * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
+                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, jobRangeInts * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dJobs, s.tableBytes ), "serve tables" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hTables, s.tableBytes, hipHostMallocDefault ), "serve tables" ) )
                 {
                     initError = fn2::LastError();
                     return false;
                 }
-                s.dCounter = (uint32_t*)( s.dKeys + (size_t)kJobRangeInts * kMaxJobs );
+                s.dCounter = (uint32_t*)( s.dKeys + (size_t)jobRangeInts * kMaxJobs );
                 s.dBlockJob = (int32_t*)( (uint8_t*)s.dJobs + jobsBytes );
                 s.dBlockBase = (uint32_t*)( (uint8_t*)s.dJobs + jobsBytes + bjBytes );
             }
@@ -287,13 +296,13 @@ namespace
                     !fn2::HipOk( hipEventCreate( &tuneEv0 ), "tune event" ) || !fn2::HipOk( hipEventCreate( &tuneEv1 ), "tune event" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneOut, (size_t)kTunePoints * 4 ), "tune arena" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneJob, sizeof( GenArgs ) ), "tune job" ) ||
-                    !fn2::HipOk( hipMalloc( (void**)&tuneKeys, kJobRangeInts * sizeof( int32_t ) ), "tune keys" ) ||
+                    !fn2::HipOk( hipMalloc( (void**)&tuneKeys, kTuneRangeInts * sizeof( int32_t ) ), "tune keys" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneBJ, tb * sizeof( int32_t ) * 2 ), "tune tables" ) )
                 {
                     initError = fn2::LastError();
                     return false;
                 }
-                deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + kJobRangeInts * 4 + tb * 8 );
+                deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + kTuneRangeInts * 4 + tb * 8 );
             }
             copies.reset( new CopyPool( 3 ) );
             dispatcher = std::thread( [this] { DispatchLoop(); } );
@@ -655,8 +664,8 @@ namespace
                 a.out = s.dOut + r->outOffset;
                 const int ppt = std::min( r->ppt, pptCap );
                 a.ppt = ppt;
-                a.minmax = s.dKeys + (size_t)kJobRangeInts * j;
-                a.rangeStride = fn2::kRangeStrideCompact;
+                a.minmax = s.dKeys + (size_t)jobRangeInts * j;
+                a.rangeStride = jobStride;
                 hJobs[j] = a;
                 const int64_t tile = (int64_t)kThreads * ppt;
                 int rb = (int)( ( r->points + tile - 1 ) / tile );
@@ -746,10 +755,10 @@ namespace
             if( mode == FN2_SERVE_PERSISTENT )
             {
                 // range keys and one zero counter per work-queue launch, in one kernel
-                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, fn2::kRangeStrideCompact, s.dCounter, (int)shape.size(), s.stream ), "range reset" ) )
+                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, jobStride, s.dCounter, (int)shape.size() * fn2::kQueueCounterInts, s.stream ), "range reset" ) )
                     return false;
             }
-            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, fn2::kRangeStrideCompact, s.stream ), "range reset" ) ) return false;
+            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, jobStride, s.stream ), "range reset" ) ) return false;
             int first = 0;
             for( size_t gi = 0; gi < shape.size(); gi++ )
             {
@@ -760,7 +769,7 @@ namespace
                 {
                     const fn2::GenArgs* jobs = s.dJobs;
                     uint32_t nTiles = (uint32_t)g[1];
-                    uint32_t* counter = s.dCounter + gi;
+                    uint32_t* counter = s.dCounter + gi * fn2::kQueueCounterInts;
                     const int64_t resident = (int64_t)cus * std::max( 1, kernels[gi]->queueBlocksPerCU );
                     const unsigned grid = (unsigned)std::min<int64_t>( g[1], resident );
                     void* params[] = { (void*)&jobs, (void*)&bj, (void*)&bb, (void*)&nTiles, (void*)&counter };
@@ -781,7 +790,7 @@ namespace
                     int dm = (int)( g[0] / 16 ), fm = (int)( g[0] % 16 );
                     if( mode == FN2_SERVE_PERSISTENT )
                     {
-                        if( !fn2::HipOk( (hipError_t)fn2_launch_generate_queue( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.dCounter + gi, s.stream ),
+                        if( !fn2::HipOk( (hipError_t)fn2_launch_generate_queue( s.dJobs, bj, bb, (int)g[1], dm, fm, (int)g[2], s.dCounter + gi * fn2::kQueueCounterInts, s.stream ),
                                          "work-queue launch" ) )
                             return false;
                     }
@@ -808,7 +817,7 @@ namespace
                     return false;
             }
             if( stagedEnd && !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, stagedEnd * 4, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
-            if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, kJobRangeInts * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
+            if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, jobRangeInts * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
             return true;
         }
 
@@ -866,7 +875,7 @@ namespace
                     copies->Run( segs );
                     for( Request* r : s.reqs )
                     {
-                        if( !fn2::DecodeRange( s.hKeys + (size_t)r->job * kJobRangeInts, fn2::kRangeStrideCompact, r->mm[0], r->mm[1] ) )
+                        if( !fn2::DecodeRange( s.hKeys + (size_t)r->job * jobRangeInts, jobStride, r->mm[0], r->mm[1] ) )
                         {
                             r->mm[0] = __builtin_inff();
                             r->mm[1] = -__builtin_inff();
