@@ -318,7 +318,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a single MI355X; "
-              "multi-GPU sharding is described in DESIGN.md but not run (got --gpus %d)" % args.gpus, file=sys.stderr)
+              "splitting the index space over several GPUs is designed but not run (got --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import fastnoise2_amd.workloads as W
