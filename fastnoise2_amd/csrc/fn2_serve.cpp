@@ -79,6 +79,7 @@ namespace
         const fn2::DeviceProgram* prog[5] = {};
         std::string compileError[5];
         int ppt[5] = {};   // tuned points per lane of the batched kernel, per dimensionality
+        bool tunedSpecialised[5] = {};   // ppt timed on the tree's specialised batch kernel
     };
 
     struct Request
@@ -424,11 +425,31 @@ namespace
             }
             if( r->ppt == 0 )
             {
-                int best = mode == FN2_SERVE_PLAIN ? 1 : Tune( prog, r->args.dims );
+                int best = mode == FN2_SERVE_PLAIN ? 1 : Tune( prog, r->args.dims, nullptr );
                 if( best <= 0 ) return -1;
                 std::lock_guard<std::mutex> lk( mu );
                 trees[treeId]->ppt[r->args.dims] = best;
                 r->ppt = best;
+            }
+            if( mode != FN2_SERVE_PLAIN )
+            {
+                // the first timing ran the interpreter; once the tree's
+                // specialised kernel is loaded, time that once more
+                const int d = r->args.dims;
+                bool retune = false;
+                {
+                    std::lock_guard<std::mutex> lk( mu );
+                    retune = !trees[treeId]->tunedSpecialised[d];
+                }
+                const fn2::JitKernel* k = retune ? fn2::GetJitKernel( prog ) : nullptr;
+                if( k )
+                {
+                    int best = Tune( prog, d, k );
+                    std::lock_guard<std::mutex> lk( mu );
+                    trees[treeId]->tunedSpecialised[d] = true;
+                    if( best > 0 ) trees[treeId]->ppt[d] = best;
+                    r->ppt = trees[treeId]->ppt[d];
+                }
             }
             r->prog = prog;
             r->args.ins = prog->dIns;
@@ -507,9 +528,10 @@ namespace
     private:
         // Per-tree tuning of the batched kernel: times a 2^20-point request of
         // the program at each candidate points-per-lane on the tuning stream
-        // and keeps the fastest. Runs once per tree and dimensionality, in the
-        // thread that submits the tree's first request of that dimensionality.
-        int Tune( const fn2::DeviceProgram* prog, int dims )
+        // and keeps the fastest. Runs in the thread that submits the tree's
+        // first request of a dimensionality (on the interpreter) and again once
+        // the tree's specialised kernel `jit` is loaded (on its batch kernel).
+        int Tune( const fn2::DeviceProgram* prog, int dims, const fn2::JitKernel* jit )
         {
             std::lock_guard<std::mutex> lk( tuneMu );
             fn2GenSpec spec{};
@@ -560,9 +582,15 @@ namespace
                 float ms = 1e30f;
                 for( int rep = 0; rep < 3; rep++ )
                 {
+                    const fn2::GenArgs* jobs = tuneJob;
+                    const int32_t* bjp = dBJ;
+                    const uint32_t* bbp = dBB;
+                    void* params[] = { (void*)&jobs, (void*)&bjp, (void*)&bbp };
                     if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, fn2::kRangeStrideCompact, tuneStream ), "tune launch" ) ||
                         !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
-                        !fn2::HipOk( (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ), "tune launch" ) ||
+                        !fn2::HipOk( jit ? hipModuleLaunchKernel( jit->batch, (unsigned)nb, 1, 1, kThreads, 1, 1, 0, tuneStream, params, nullptr )
+                                         : (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ),
+                                     "tune launch" ) ||
                         !fn2::HipOk( hipEventRecord( tuneEv1, tuneStream ), "tune event" ) ||
                         !fn2::HipOk( hipEventSynchronize( tuneEv1 ), "tune run" ) )
                         return -1;
