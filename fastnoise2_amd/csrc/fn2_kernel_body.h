@@ -264,12 +264,14 @@ namespace fn2k
     // batch that fits the grid takes no atomic at all, and a large one
     // spreads its pulls over as many addresses (one counter for every
     // wave queued up at about 5 ns a pull, nearly doubling a large batch).
-    // A wave keeps a running range of the job it is on, published when the
-    // job changes and at the end. The loop has no barrier and no branch that
-    // only some lanes take around its back edge: all lanes take part in the
-    // pull (lane 0 adds one, the others zero) and in publishing a range. The
-    // counters only grow, so every wave leaves after at most a part's items
-    // + 1 pulls and no wave waits for another.
+    // A wave publishes each item's range to its slot of the job's record
+    // (a running range across items, published on a change of job, kept 8
+    // more VGPRs live in the loop: a wave per SIMD on the serving tree). The
+    // loop has no barrier and no branch that only some lanes take around its
+    // back edge: all lanes take part in the pull (lane 0 adds one, the
+    // others zero) and in publishing a range. The counters only grow, so
+    // every wave leaves after at most a part's items + 1 pulls and no wave
+    // waits for another.
     template<int BLOCK, class Prog>
     __device__ __forceinline__ void generateQueue( const fn2::GenArgs* __restrict__ jobs, const int32_t* __restrict__ tileJob,
                                                    const uint32_t* __restrict__ tileBase, uint32_t nTiles, uint32_t* counter, Prog prog )
@@ -277,9 +279,6 @@ namespace fn2k
         constexpr uint32_t kWaves = BLOCK / 64;
         const uint32_t lane = threadIdx.x & 63;
         const uint32_t items = nTiles * kWaves;
-        int32_t* keys = nullptr;   // the record of the job whose range mn, mx hold
-        int keyStride = 0;
-        float mn = __builtin_inff(), mx = -__builtin_inff();
         const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane( (int)( blockIdx.x * kWaves + ( threadIdx.x >> 6 ) ) );
         const uint32_t waves = gridDim.x * kWaves;
         const int slot = (int)( g % fn2::kRangeSlots );
@@ -294,14 +293,7 @@ namespace fn2k
         {
             const uint32_t t = w / kWaves;
             const fn2::GenArgs& a = jobs[tileJob[t]];
-            if( a.minmax != keys )
-            {
-                waveMinMax( mn, mx, keys, slot, keyStride );
-                keys = a.minmax;
-                keyStride = a.rangeStride;
-                mn = __builtin_inff();
-                mx = -__builtin_inff();
-            }
+            float mn = __builtin_inff(), mx = -__builtin_inff();
             const uint32_t total = (uint32_t)a.total;
             const uint32_t base = tileBase[t] + ( w % kWaves ) * 64 + lane;
             const int ppt = a.ppt;
@@ -341,11 +333,11 @@ namespace fn2k
                     }
                 }
             }
+            waveMinMax( mn, mx, a.minmax, slot, a.rangeStride );
             if( first >= end ) break;   // no items in this wave's part
             const uint32_t pulled = atomicAdd( myCounter, lane == 0 ? 1u : 0u );
             w = first + (uint32_t)__builtin_amdgcn_readfirstlane( (int)pulled );
             if( w >= end ) break;
         }
-        waveMinMax( mn, mx, keys, slot, keyStride );
     }
 }
