@@ -29,100 +29,8 @@ import oracle
 from fastnoise2_amd.tree import NodeSpec as S
 from conftest import assert_bitexact, assert_close
 
-PX, PY, PZ, PW = 0xF797C5C7, 0x6C060C89, 0x465FD04F, 0xF7A62279
-HASH_A = 0xB7E0A5F5
-R2F = float(np.float32(math.sqrt(2.0)))
-P2 = float(np.float32(np.float32(1.0) + np.float32(R2F)))   # 1 + kRoot2f in float
-
-# GetGradientDotPerlin 2D, AVX2 table (Utils.inl:174-180)
-G2X = [P2, -P2, P2, -P2, 1, -1, 1, -1]
-G2Y = [1, 1, -1, -1, P2, P2, -P2, -P2]
-# GetGradientDotCommon 3D, AVX-512 table (Utils.inl:35-37)
-G3X = [1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, 0, -1, 0]
-G3Y = [1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1]
-G3Z = [0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 0, 1, 0, -1]
-# GetGradientDotPerlin 4D, AVX-512 two-table permute (Utils.inl:191-196): index 0..15 from the
-# first table, 16..31 from the second
-G4X = [0] * 8 + [1, -1] * 4 + [1, -1] * 8
-G4Y = [1, -1] * 4 + [0] * 8 + [1, 1, -1, -1] * 4
-G4Z = [1, 1, -1, -1] * 4 + [0] * 8 + [1, 1, 1, 1, -1, -1, -1, -1]
-G4W = [1, 1, 1, 1, -1, -1, -1, -1] * 2 + [1, 1, 1, 1, -1, -1, -1, -1] + [0] * 8
-
-
-def i32(v):
-    v = np.asarray(v, dtype=np.int64) & 0xFFFFFFFF
-    return np.where(v >= 1 << 31, v - (1 << 32), v)
-
-
-def hash_primes(seed, *primed):
-    h = np.int64(seed)
-    for p in primed:
-        h = h ^ p
-    h = i32(h * HASH_A)
-    return i32((h >> 15) ^ h)
-
-
-def value_coord(seed, *primed):
-    h = np.int64(seed)
-    for p in primed:
-        h = h ^ p
-    h = i32(h * i32(h * HASH_A))
-    return h.astype(np.float64)
-
-
-def quintic(t):
-    return t * t * t * (t * (t * 6 - 15) + 10)
-
-
-def hermite(t):
-    return t * t * (3 - 2 * t)
-
-
-def lerp(a, b, t):
-    return a + t * (b - a)
-
-
-def scale_output(v, nmin, nmax, rmin=-1.0, rmax=1.0):
-    return (v - nmin) / (nmax - nmin) * (rmax - rmin) + rmin
-
-
-def perlin_ref(pos, seed, freq):
-    """float64 Perlin 2D/3D/4D written from Perlin.inl with table gradients."""
-    d = len(pos)
-    p = [np.asarray(c, np.float64) * np.float64(np.float32(freq)) for c in pos]
-    fl = [np.floor(c) for c in p]
-    base = [i32(f.astype(np.int64) * pr) for f, pr in zip(fl, (PX, PY, PZ, PW))]
-    fr = [c - f for c, f in zip(p, fl)]
-    t = [quintic(f) for f in fr]
-    tabs = {2: (G2X, G2Y), 3: (G3X, G3Y, G3Z), 4: (G4X, G4Y, G4Z, G4W)}[d]
-    mask = {2: 7, 3: 15, 4: 31}[d]
-    corner = {}
-    for c in range(1 << d):
-        prim = [i32(base[a] + (((c >> a) & 1) * (PX, PY, PZ, PW)[a])) for a in range(d)]
-        h = hash_primes(seed, *prim) & mask
-        off = [fr[a] - ((c >> a) & 1) for a in range(d)]
-        corner[c] = sum(np.asarray(tabs[a])[h] * off[a] for a in range(d))
-    vals = [corner[c] for c in range(1 << d)]
-    for a in range(d):
-        vals = [lerp(vals[2 * k], vals[2 * k + 1], t[a]) for k in range(len(vals) // 2)]
-    bound = {2: 1.726796627044677734375, 3: 1.0363423824310302734375, 4: 1.33858621120452880859375}[d]
-    return scale_output(vals[0], -bound, bound)
-
-
-def value_ref(pos, seed, freq):
-    """float64 Value noise 2D/3D/4D written from Value.inl."""
-    d = len(pos)
-    p = [np.asarray(c, np.float64) * np.float64(np.float32(freq)) for c in pos]
-    fl = [np.floor(c) for c in p]
-    base = [i32(f.astype(np.int64) * pr) for f, pr in zip(fl, (PX, PY, PZ, PW))]
-    t = [hermite(c - f) for c, f in zip(p, fl)]
-    vals = []
-    for c in range(1 << d):
-        prim = [i32(base[a] + (((c >> a) & 1) * (PX, PY, PZ, PW)[a])) for a in range(d)]
-        vals.append(value_coord(seed, *prim))
-    for a in range(d):
-        vals = [lerp(vals[2 * k], vals[2 * k + 1], t[a]) for k in range(len(vals) // 2)]
-    return scale_output(vals[0], -2147483648.0, 2147483648.0)
+from f64ref import (PX, PY, PZ, PW, HASH_A, G2X, G2Y, G3X, G3Y, G3Z, G4X, G4Y, G4Z, G4W, i32, hash_primes,
+                    value_coord, quintic, hermite, scale_output, perlin_ref, value_ref)
 
 
 def rand_points(d, n, lo=-300.0, hi=300.0, seed=5):
