@@ -67,6 +67,8 @@ def load():
     L.fn2o_grad_dot.argtypes = [i, i, _fp, i]
     L.fn2o_interp.restype = f
     L.fn2o_interp.argtypes = [i, f, i]
+    L.fn2o_warp_vec.restype = None
+    L.fn2o_warp_vec.argtypes = [i, i, i, _fp, f, _fp, i]
     L.fn2o_base64_decode.restype = i
     L.fn2o_base64_decode.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i]
     _lib = L

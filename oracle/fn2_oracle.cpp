@@ -496,6 +496,24 @@ float fn2o_interp( int kind, float t, int mode )
     return kind == 0 ? Eval<1, true, false>::interpHermite( x ).v[0] : Eval<1, true, false>::interpQuintic( x ).v[0];
 }
 
+void fn2o_warp_vec( int dims, int scheme, int hash, const float* f, float mult, float* v, int mode )
+{
+    using F1 = FV<1>;
+    F1 fv[4], vv[4];
+    for( int a = 0; a < dims; a++ ) { fv[a] = fsplat<1>( f[a] ); vv[a] = fsplat<1>( v[a] ); }
+    auto h = isplat<1>( hash );
+    auto m = fsplat<1>( mult );
+    auto run = [&]( auto eval ) {
+        using E = decltype( eval );
+        if( dims == 2 ) E::template vecContribution<2>( scheme, h, fv, m, vv );
+        else if( dims == 3 ) E::template vecContribution<3>( scheme, h, fv, m, vv );
+        else E::template vecContribution<4>( scheme, h, fv, m, vv );
+    };
+    if( mode & FN2O_UNFUSED ) run( Eval<1, false, false>() );
+    else run( Eval<1, true, false>() );
+    for( int a = 0; a < dims; a++ ) v[a] = vv[a].v[0];
+}
+
 int fn2o_base64_decode( const char* in, unsigned char* out, int cap )
 {
     std::vector<uint8_t> v;

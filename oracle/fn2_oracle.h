@@ -50,6 +50,11 @@ int fn2o_hash_primes_hb( int seed, int n, const int* primed );   /* HashPrimesHB
 float fn2o_value_coord( int seed, int n, const int* primed );    /* GetValueCoord */
 float fn2o_grad_dot( int dims, int hash, const float* f, int mode ); /* GetGradientDotPerlin 2D/4D, Common 3D */
 float fn2o_interp( int kind, float t, int mode );                /* 0 Hermite, 1 Quintic */
+/* One vertex's vector contribution of the simplex-type warps
+ * (DomainWarpSimplex.inl:6-724): scheme 0 OrthogonalGradientMatrix,
+ * 1 GradientOuterProduct; v[0..dims) accumulates mult * (vector for hash
+ * applied to offset f). */
+void fn2o_warp_vec( int dims, int scheme, int hash, const float* f, float mult, float* v, int mode );
 
 /* Base64 with the '@' run-length extension (Base64.h). Returns decoded length or -1. */
 int fn2o_base64_decode( const char* in, unsigned char* out, int cap );

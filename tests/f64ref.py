@@ -424,3 +424,249 @@ def warp_fractal_displacement(pos, seed, octaves, gain, lacunarity, weighted_str
         cur = [cur[a] + vec[a] * amp * axis_scale[a] for a in range(d)]
         prev_len = np.sqrt(strength_sq)
     return [cur[a] - orig[a] for a in range(d)]
+
+
+# --------------------------------------------------------------- simplex-warp vectors
+# One vertex's contribution to the simplex-type warps' vector: multiplier *
+# (vector chosen by the hash, applied to the offset f). Restated from the
+# table (AVX-512) branches of DomainWarpSimplex.inl:6-724; the oracle follows
+# the generic branches, so these pin its bit logic. Table constants are
+# float32 values formed in float32 arithmetic as the reference's constexpr
+# floats are.
+def _u32(v):
+    return np.asarray(v, np.int64) & 0xFFFFFFFF
+
+
+def _s32(v):
+    v = _u32(v)
+    return np.where(v >= 1 << 31, v - (1 << 32), v)
+
+
+def _ff(*xs):
+    """Sum of float32 values in float32 arithmetic, left to right."""
+    acc = f32(xs[0])
+    for x in xs[1:]:
+        acc = f32(acc + f32(x))
+    return float(acc)
+
+
+_SKEW2F = float(f32(1.0 / (math.sqrt(3.0) + 1.0)))
+_SKEW4F = float(f32(1.0 / (math.sqrt(5.0) + 1.0)))
+_ROOT3F = float(f32(math.sqrt(3.0)))
+_ROOT2F = float(f32(math.sqrt(2.0)))
+
+
+def _perm2(t1, t2, idx):
+    """_mm512_permutex2var_ps: index bit 4 chooses the second table."""
+    idx = np.asarray(idx) & 31
+    tab = np.array(list(t1) + list(t2), np.float64)
+    return tab[idx]
+
+
+def _ortho2(h, f, m):
+    s = _SKEW2F
+    s1 = _ff(s, 1)
+    tx = [s, -s, s, -s, s1, -s1, s1, -s1, 1, -1, 1, -1, 0, 0, 0, 0]
+    ty = [s1, s1, -s1, -s1, s, s, -s, -s, 1, 1, -1, -1, 0, 0, 0, 0]
+    idx = _u32((_u32(h) >> 1) * 3) >> 28
+    gx, gy = np.array(tx)[idx], np.array(ty)[idx]
+    sign = np.where(_u32(h) & 1, -1.0, 1.0)
+    return [m * (f[1] * gy + f[0] * gx), m * sign * (f[1] * gx - f[0] * gy)]
+
+
+def _outer_index(hs):
+    iov = _u32(hs * 0xAAAAAAAB) & 0xC0000003
+    return (iov & 3) | ((iov >> 30) << 2)
+
+
+def _outer2(h, f, m):
+    r = _ROOT3F
+    tx = [r, -r, 1, -1, r, -r, -1, 1, 2, -2, 0, 0, 0, 0, 0, 0]
+    ty = [1, -1, r, -r, -1, 1, r, -r, 0, 0, 2, -2, 0, 0, 0, 0]
+    hs = _u32(h) >> 1
+    ig = _u32(hs * 3) >> 28
+    io = _outer_index(hs)
+    mm = m * (f[1] * np.array(ty)[ig] + f[0] * np.array(tx)[ig])
+    return [mm * np.array(tx)[io], mm * np.array(ty)[io]]
+
+
+def _outer3(h, f, m):
+    tx = [1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0]
+    ty = [1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0]
+    tz = [0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1, 0, 0, 0, 0]
+    hs = _u32(h) >> 1
+    ig = _u32(hs * 3) >> 28
+    io = _outer_index(hs)
+    t = [np.array(tx), np.array(ty), np.array(tz)]
+    mm = m * sum(t[a][ig] * f[a] for a in range(3))
+    return [mm * t[a][io] for a in range(3)]
+
+
+def _outer4(h, f, m):
+    s, s1 = _SKEW4F, _ff(_SKEW4F, 1)
+    tabs = [[s1, s, s, s, -1, 1, 0, 0, -1, 0, 1, 0, -1, 0, 0, 1],
+            [s, s1, s, s, 1, -1, 0, 0, 0, -1, 0, 1, 0, -1, 1, 0],
+            [s, s, s1, s, 0, 0, -1, 1, 1, 0, -1, 0, 0, 1, -1, 0],
+            [s, s, s, s1, 0, 0, 1, -1, 0, 1, 0, -1, 1, 0, 0, -1]]
+    hs = _u32(h) >> 2
+    ig = _u32(hs * 5) >> 27
+    iov = _u32(hs * 0xCCCCCCCD)
+    io = _u32((iov & 0xE0000003) * (3 | (1 << 27))) >> 27
+    g = [_perm2(t, [-x for x in t], ig) for t in tabs]
+    o = [_perm2(t, [-x for x in t], io) for t in tabs]
+    mm = m * sum(g[a] * f[a] for a in range(4))
+    return [mm * o[a] for a in range(4)]
+
+
+def _ortho3(h, f, m):
+    A, B, C, DE, F = 2.224744871391589, -0.224744871391589, -1.0, 1.0, 2.0
+    A, B = float(f32(A)), float(f32(B))
+    tA = ([A, A, C, C, -A, -A, C, C, A, A, C, C, -A, -A, C, C],
+          [C, B, A, A, C, B, -A, -A, C, -B, A, A, C, -B, -A, -A],
+          [B, C, B, B, B, C, B, B, -B, C, -B, -B, -B, C, -B, -B])
+    tB = ([B, B, C, C, -B, -B, C, C, B, B, C, C, -B, -B, C, C],
+          [C, A, B, B, C, A, -B, -B, C, -A, B, B, C, -A, -B, -B],
+          [A, C, A, A, A, C, A, A, -A, C, -A, -A, -A, C, -A, -A])
+    tC = ([DE, DE, F, F, C, C, F, F, DE, DE, F, F, C, C, F, F],
+          [F, DE, DE, DE, F, DE, C, C, F, C, DE, DE, F, C, C, C],
+          [DE, F, DE, DE, DE, F, DE, DE, C, F, C, C, C, F, C, C])
+    hs = _u32(h) >> 1
+    ifb = _u32(hs * 0xAAAAAAAB)
+    rot = _u32((ifb << 2) | (ifb >> 30))
+    ip3 = _u32(hs * 3) >> 30
+    vab = sum(_perm2(tA[a], tB[a], rot) * f[a] for a in range(3))
+    vba = sum(_perm2(tB[a], tA[a], rot) * f[a] for a in range(3))
+    vc = sum(np.array(tC[a])[rot & 15] * f[a] for a in range(3))
+    mm = m * np.where(_u32(h) & 1, -1.0, 1.0)
+    return [mm * np.where(ip3 == 0, vc, vab),
+            mm * np.where(ip3 == 1, vc, np.where(ip3 == 2, vba, vab)),
+            mm * np.where(ip3 == 2, vc, vba)]
+
+
+def _ortho4_tables():
+    kNegAB, kFillAB = float(f32(-0.375999676691291)), float(f32(0.222726847849776))
+    kPosD, kUnD = -_SKEW4F, _SKEW4F
+    dPSAB, dPSD = float(f32(-0.124000323308709)), float(f32(0.190983005625053))
+    dSE, dABC, dFD = _SKEW4F, float(f32(0.437016024448821)), -_ROOT2F
+    dPSEAB, dPSED = _ff(dPSAB, dSE), _ff(dPSD, dSE)
+    sIdxABC, sDiagABC, sFillABC = _ff(kNegAB, dPSAB), _ff(kFillAB, dPSAB, dFD), _ff(kFillAB, dPSAB)
+    sIdxD, sFillD = _ff(kPosD, dPSD, -1), _ff(kUnD, dPSD)
+    pIdxPosAB, pIdxNegAB, pFillAB, pDiagAB = _ff(kNegAB, 1), kNegAB, kFillAB, _ff(kFillAB, dFD)
+    pIdxPosC, pIdxNegC, pFillC = _ff(kNegAB, dABC, 1), _ff(kNegAB, dABC), _ff(kFillAB, dABC)
+    pIdxPosD, pIdxNegD, pFillD = kPosD, _ff(kPosD, -1), kUnD
+    eIdxABC, eDiagABC, eFillABC = _ff(kNegAB, dPSEAB, 1), _ff(kFillAB, dPSEAB, dFD), _ff(kFillAB, dPSEAB)
+    eIdxD, eFillD = _ff(kPosD, dPSED), _ff(kUnD, dPSED)
+    z = [0] * 12
+    return {
+        "A_gX": ([sIdxABC, sDiagABC, sDiagABC, sDiagABC, pIdxPosAB, pIdxNegAB, pDiagAB, pDiagAB, pIdxPosAB, pDiagAB, pIdxNegAB, pDiagAB, pIdxPosAB, pDiagAB, pDiagAB, pIdxNegAB], [eIdxABC, eDiagABC, eDiagABC, eDiagABC] + z),
+        "B_gX": ([sIdxABC, sFillABC, sFillABC, sFillABC, pIdxPosAB, pIdxNegAB, pFillAB, pFillAB, pIdxPosAB, pFillAB, pIdxNegAB, pFillAB, pIdxPosAB, pFillAB, pFillAB, pIdxNegAB], [eIdxABC, eFillABC, eFillABC, eFillABC] + z),
+        "C_gX": ([sIdxABC, sFillABC, sFillABC, sFillABC, pIdxPosC, pIdxNegC, pFillC, pFillC, pIdxPosC, pFillC, pIdxNegC, pFillC, pIdxPosC, pFillC, pFillC, pIdxNegC], [eIdxABC, eFillABC, eFillABC, eFillABC] + z),
+        "D_gX": ([sIdxD, sFillD, sFillD, sFillD, pIdxPosD, pIdxNegD, pFillD, pFillD, pIdxPosD, pFillD, pIdxNegD, pFillD, pIdxPosD, pFillD, pFillD, pIdxNegD], [eIdxD, eFillD, eFillD, eFillD] + z),
+        "A_gY": ([sDiagABC, sIdxABC, sFillABC, sFillABC, pIdxNegAB, pIdxPosAB, pFillAB, pFillAB, pDiagAB, pIdxPosAB, pDiagAB, pIdxNegAB, pDiagAB, pIdxPosAB, pIdxNegAB, pDiagAB], [eDiagABC, eIdxABC, eFillABC, eFillABC] + z),
+        "B_gY": ([sFillABC, sIdxABC, sDiagABC, sDiagABC, pIdxNegAB, pIdxPosAB, pDiagAB, pDiagAB, pFillAB, pIdxPosAB, pFillAB, pIdxNegAB, pFillAB, pIdxPosAB, pIdxNegAB, pFillAB], [eFillABC, eIdxABC, eDiagABC, eDiagABC] + z),
+        "C_gY": ([sFillABC, sIdxABC, sFillABC, sFillABC, pIdxNegC, pIdxPosC, pFillC, pFillC, pFillC, pIdxPosC, pFillC, pIdxNegC, pFillC, pIdxPosC, pIdxNegC, pFillC], [eFillABC, eIdxABC, eFillABC, eFillABC] + z),
+        "D_gY": ([sFillD, sIdxD, sFillD, sFillD, pIdxNegD, pIdxPosD, pFillD, pFillD, pFillD, pIdxPosD, pFillD, pIdxNegD, pFillD, pIdxPosD, pIdxNegD, pFillD], [eFillD, eIdxD, eFillD, eFillD] + z),
+        "A_gZ": ([sFillABC, sFillABC, sIdxABC, sFillABC, pDiagAB, pDiagAB, pIdxPosAB, pIdxNegAB, pIdxNegAB, pFillAB, pIdxPosAB, pFillAB, pFillAB, pIdxNegAB, pIdxPosAB, pFillAB], [eFillABC, eFillABC, eIdxABC, eFillABC] + z),
+        "B_gZ": ([sDiagABC, sDiagABC, sIdxABC, sFillABC, pFillAB, pFillAB, pIdxPosAB, pIdxNegAB, pIdxNegAB, pDiagAB, pIdxPosAB, pDiagAB, pDiagAB, pIdxNegAB, pIdxPosAB, pDiagAB], [eDiagABC, eDiagABC, eIdxABC, eFillABC] + z),
+        "C_gZ": ([sFillABC, sFillABC, sIdxABC, sDiagABC, pFillC, pFillC, pIdxPosC, pIdxNegC, pIdxNegC, pFillC, pIdxPosC, pFillC, pFillC, pIdxNegC, pIdxPosC, pFillC], [eFillABC, eFillABC, eIdxABC, eDiagABC] + z),
+        "D_gZ": ([sFillD, sFillD, sIdxD, sFillD, pFillD, pFillD, pIdxPosD, pIdxNegD, pIdxNegD, pFillD, pIdxPosD, pFillD, pFillD, pIdxNegD, pIdxPosD, pFillD], [eFillD, eFillD, eIdxD, eFillD] + z),
+        "A_gW": ([sFillABC, sFillABC, sFillABC, sIdxABC, pFillAB, pFillAB, pIdxNegAB, pIdxPosAB, pFillAB, pIdxNegAB, pFillAB, pIdxPosAB, pIdxNegAB, pFillAB, pFillAB, pIdxPosAB], [eFillABC, eFillABC, eFillABC, eIdxABC] + z),
+        "B_gW": ([sFillABC, sFillABC, sFillABC, sIdxABC, pDiagAB, pDiagAB, pIdxNegAB, pIdxPosAB, pDiagAB, pIdxNegAB, pDiagAB, pIdxPosAB, pIdxNegAB, pDiagAB, pDiagAB, pIdxPosAB], [eFillABC, eFillABC, eFillABC, eIdxABC] + z),
+        "C_gW": ([sDiagABC, sDiagABC, sDiagABC, sIdxABC, pFillC, pFillC, pIdxNegC, pIdxPosC, pFillC, pIdxNegC, pFillC, pIdxPosC, pIdxNegC, pFillC, pFillC, pIdxPosC], [eDiagABC, eDiagABC, eDiagABC, eIdxABC] + z),
+        "D_gW": ([sFillD, sFillD, sFillD, sIdxD, pFillD, pFillD, pIdxNegD, pIdxPosD, pFillD, pIdxNegD, pFillD, pIdxPosD, pIdxNegD, pFillD, pFillD, pIdxPosD], [eFillD, eFillD, eFillD, eIdxD] + z),
+    }
+
+
+_ORTHO4 = None
+
+
+def _ortho4(h, f, m):
+    global _ORTHO4
+    if _ORTHO4 is None:
+        _ORTHO4 = _ortho4_tables()
+    hs = _u32(h) >> 2
+    basis = _u32(hs * 5)
+    idx = basis >> 27
+    ip3 = _s32(hs * 0xD5555556) >> 29          # arithmetic shift of the int32
+    ip8 = _s32(basis) >> 24
+    vals = {k: sum(_perm2(*_ORTHO4["%s_g%s" % (k, ax)], idx) * f[a] for a, ax in enumerate("XYZW")) for k in "ABCD"}
+    ix = ip8 >> 1
+    iy = ix + 1 + ip3
+    iz0 = ip8 & 1
+    iz = ix + 1 + np.where(iz0 >= ip3, iz0 + 1, iz0)
+    isum = ix + iy + iz
+    pick = lambda i, order: np.choose(np.asarray(i) & 3, [vals[k] for k in order])
+    mm = m * np.where(_u32(h) & 1, -1.0, 1.0)
+    return [mm * pick(ix, "ABCD"), mm * pick(iy, "ABCD"), mm * pick(iz, "ABCD"), mm * pick(isum, "CBAD")]
+
+
+WARP_VECTOR = {(2, 0): _ortho2, (3, 0): _ortho3, (4, 0): _ortho4, (2, 1): _outer2, (3, 1): _outer3, (4, 1): _outer4}
+
+# Output bounding of the simplex-type warps (DomainWarpSimplex.inl:823-825,
+# 919-921, 1073-1075, 1208-1210, 1434-1436, 1821-1823): (scheme 0, scheme 1)
+WARP_BOUNDING = {
+    ("Simplex", 2): (70.1480577066486, 49.918426513671875 / 2.0),
+    ("Simplex", 3): (16.281631889139874, 32.69428253173828125 / 1.4142135623730951),
+    ("Simplex", 4): (30.88161777516092, 33.653125584827855 / 1.4142135623730951),
+    ("SuperSimplex", 2): (12.814453124999995, 9.28993664146183 / 2.0),
+    ("SuperSimplex", 3): (37.63698669623629, 144.736422163332608 / 1.4142135623730951),
+    ("SuperSimplex", 4): (48.80058117543753, 115.21625311930542 / 1.4142135623730951),
+}
+
+
+def warp_simplex_vector(name, pos, seed, freq, scheme, helper=None):
+    """DomainWarpSimplex / SuperSimplex's warp vector before amplitude and axis
+    scale: bounding * sum over the node's lattice vertices of
+    kernel(|d|^2) * vector(HashPrimes(L), d), over the same vertex sets and
+    kernels as the noise nodes. `seed` is the warp's hash seed: the call's
+    seed plus twice the node's seed offset for DomainWarpSimplex (added in
+    DomainWarp::GenT and again in its Warp), once for DomainWarpSuperSimplex,
+    whose Warp adds none (DomainWarpSimplex.inl:731-766, 1094-1128).
+    Returns (vector per axis, |vector|^2 at unit axis scale)."""
+    d = len(pos)
+    r2, ks, _ = SIMPLEX_KERNEL[(name, d)]
+    bound = float(f32(WARP_BOUNDING[(name, d)][scheme]))
+    p = scale_pos(pos, freq)
+    acc = [np.zeros_like(p[0]) for _ in range(d)]
+    for L, dv in lattice_points(p, d, VERTEX_SET[(name, d)]):
+        q = sum(c * c for c in dv)
+        fall = np.where(q < r2, np.maximum(0.0, ks * (r2 - q)) ** 4, 0.0)
+        v = (helper or WARP_VECTOR[(d, scheme)])(hash_primes(seed, *primed(L)), dv, fall)
+        for a in range(d):
+            acc[a] += v[a]
+    if d == 3 and scheme == 1:
+        # the outer-product vectors are turned to the 3D lattice's reflected
+        # offsets: v += k (v.1) 1 with k = -1 (Simplex) or -2/3 (SuperSimplex)
+        # (DomainWarpSimplex.inl:909-917, 1424-1432)
+        k = float(f32(-1.0)) if name == "Simplex" else float(f32(-2.0 / 3.0))
+        t = k * sum(acc)
+        acc = [a + t for a in acc]
+    vec = [a * bound for a in acc]
+    return vec, sum(v * v for v in vec)
+
+
+def warp_table_domain(d, scheme, h):
+    """Hashes for which the reference's table branches and generic branch of
+    a simplex-warp vector helper compute the same vector: every table index
+    inside the range the tables were laid out for ([0,12) gradient and outer
+    indices, [0,3) permutation selectors in 2D/3D; [0,20) basis indices in
+    4D) and, for the 4D orthogonal matrix, the pairwise basis group [4,16)
+    only (its generic branch compares the basis index against 0x10 << 27 and
+    0x04 << 27 as signed integers, which folds the single and extra groups
+    together). Outside this domain the reference's SIMD levels disagree with
+    each other; the oracle follows the generic branch."""
+    u = _u32(h)
+    hs1, hs2 = u >> 1, u >> 2
+    if d in (2, 3):
+        grad_ok = (_u32(hs1 * 3) >> 28) < 12
+        outer_ok = (_u32(hs1 * 0xAAAAAAAB) >> 30) < 3
+        if (d, scheme) == (2, 0):
+            return grad_ok
+        if scheme == 1:
+            return grad_ok & outer_ok
+        return outer_ok & ((_u32(hs1 * 3) >> 30) < 3)
+    basis = _u32(hs2 * 5) >> 27
+    if scheme == 0:
+        return (basis >= 4) & (basis < 16)
+    outer = _u32((_u32(hs2 * 0xCCCCCCCD) & 0xE0000003) * (3 | (1 << 27))) >> 27
+    return (basis < 20) & (outer < 20)

@@ -284,3 +284,68 @@ def test_warp_fractal_displacement(d, kind):
              **{"amplitude_scaling_" + "xyzw"[k]: sc[k] for k in range(d)})
     got = oracle_positions(S(kind, domain_warp_source=warp, octaves=4, gain=0.6, lacunarity=2.0), pos, 21).astype(np.float64) - pos[0]
     assert np.abs(got - other[0]).max() > 1e-3
+
+
+# ---------------------------------------------------------------- simplex-type warps
+def _probe_vectors(d, scheme):
+    """The oracle's one-vertex warp vector helper as a vectorised callable
+    (hashes, offsets per axis, multipliers) -> vectors per axis."""
+    L = oracle.load()
+
+    def helper(hashes, f, m):
+        hashes = np.asarray(hashes)
+        F = np.ascontiguousarray(np.stack(f, 1).astype(np.float32))
+        M = np.broadcast_to(np.asarray(m, np.float32), hashes.shape)
+        out = np.zeros((hashes.shape[0], d))
+        v = np.zeros(d, np.float32)
+        for k in range(hashes.shape[0]):
+            if M[k] == 0:
+                continue
+            v[:] = 0
+            fk = np.ascontiguousarray(F[k])
+            L.fn2o_warp_vec(d, scheme, int(hashes[k]), fk.ctypes.data_as(oracle._fp), float(M[k]),
+                            v.ctypes.data_as(oracle._fp), 0)
+            out[k] = v
+        return [out[:, a] for a in range(d)]
+    return helper
+
+
+@pytest.mark.parametrize("d", [2, 3, 4])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_warp_vector_helpers_match_table_branches(d, scheme):
+    # the oracle follows the generic branches of the helpers; the table
+    # (AVX-512) branches are an independent formulation of the same vectors
+    # wherever the table indices stay inside their laid-out ranges
+    rng = np.random.default_rng(40 + d * 2 + scheme)
+    n = 6000
+    hashes = rng.integers(-2 ** 31, 2 ** 31, n)
+    f = rng.uniform(-1, 1, (n, d))
+    m = rng.uniform(0.1, 2.0, n)
+    got = np.stack(_probe_vectors(d, scheme)(hashes, [f[:, a] for a in range(d)], m), 1)
+    ref = np.stack(R.WARP_VECTOR[(d, scheme)](hashes, [f[:, a].astype(np.float32).astype(np.float64) for a in range(d)],
+                                              m.astype(np.float32).astype(np.float64)), 1)
+    dom = R.warp_table_domain(d, scheme, hashes)
+    assert dom.mean() > 0.35
+    assert np.abs(got - ref)[dom].max() < 1e-5, (d, scheme)
+    # outside the domain the reference's branches disagree with each other;
+    # the difference is a different vector, not rounding
+    assert (np.abs(got - ref).max(1)[~dom] > 1e-3).mean() > 0.5
+
+
+@pytest.mark.parametrize("name", ["DomainWarpSimplex", "DomainWarpSuperSimplex"])
+@pytest.mark.parametrize("d", [2, 3, 4])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_simplex_warp_lattice_sum(name, d, scheme):
+    # displacement = amplitude * axis scale * bounding * sum over the node's
+    # lattice vertices of kernel * vector(hash, offset), the per-vertex vector
+    # from the oracle's (separately pinned) helper
+    pos = points(d, 1200, -30.0, 30.0, 11)
+    sc = [1.0, 0.5, 2.0, 1.5][:d]
+    # DomainWarpSimplex adds its seed offset twice, DomainWarpSuperSimplex once
+    warp_seed = 11 + (6 if name == "DomainWarpSimplex" else 3)
+    vec, _ = R.warp_simplex_vector(name[len("DomainWarp"):], pos, warp_seed, 1 / 7.0, scheme, _probe_vectors(d, scheme))
+    for a in range(d):
+        spec = S(name, feature_scale=7.0, warp_amplitude=1.5, seed_offset=3, vectorization_scheme=scheme,
+                 source=_probe(a), **{"amplitude_scaling_" + "xyzw"[k]: sc[k] for k in range(d)})
+        got = oracle_positions(spec, pos, 11).astype(np.float64) - pos[a]
+        assert np.abs(got - 1.5 * sc[a] * vec[a]).max() < 1e-5, (name, d, scheme, a)
