@@ -186,3 +186,75 @@ def op_counts_per_point(encoded, dims, offset, count, step=None, seed=1337, tile
     L.fn2o_counters(ctypes.byref(flop), ctypes.byref(valu))
     L.fn2o_delete(h)
     return flop.value / n, valu.value / n
+
+
+TRACE_LIB_PATH = os.path.join(HERE, "_build", "libfn2_oracle_trace.so")
+_trace_lib = None
+
+
+def _load_trace():
+    global _trace_lib
+    if _trace_lib is None:
+        if not os.path.exists(TRACE_LIB_PATH):
+            raise ImportError("index-replay oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(TRACE_LIB_PATH)
+        vp, f, i = ctypes.c_void_p, ctypes.c_float, ctypes.c_int
+        L.fn2o_new_from_encoded.restype = vp
+        L.fn2o_new_from_encoded.argtypes = [ctypes.c_char_p]
+        L.fn2o_delete.argtypes = [vp]
+        L.fn2o_set_trace.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
+        tail = [i, i, i, i, _fp]
+        L.fn2o_gen_grid2d.argtypes = [vp, _fp, f, f, i, i, f, f] + tail
+        L.fn2o_gen_grid3d.argtypes = [vp, _fp, f, f, f, i, i, i, f, f, f] + tail
+        L.fn2o_gen_grid4d.argtypes = [vp, _fp, f, f, f, f, i, i, i, i, f, f, f, f] + tail
+        L.fn2o_gen_pos2d.argtypes = [vp, _fp, i, _fp, _fp, f, f] + tail
+        L.fn2o_gen_pos3d.argtypes = [vp, _fp, i, _fp, _fp, _fp, f, f, f] + tail
+        L.fn2o_gen_pos4d.argtypes = [vp, _fp, i, _fp, _fp, _fp, _fp, f, f, f, f] + tail
+        L.fn2o_gen_tileable2d.argtypes = [vp, _fp, i, i, f, f] + tail
+        _trace_lib = L
+    return _trace_lib
+
+
+def index_trace(encoded, kind, mode=FUSED, seed=1337, offset=None, count=None, step=None, pos=None):
+    """Values and per-point decision fingerprints of one call in the oracle's
+    index-replay build: kind "grid" (offset, count, step), "positions" (pos
+    arrays, offset) or "tileable" (count = (x, y), step). Two variants of the
+    oracle agree on a point's fingerprint exactly when every float -> int
+    conversion, float comparison, floor / round result and hashed position
+    bit pattern on its path agrees: the same lattice cells, simplex vertices,
+    cells and selections."""
+    L = _load_trace()
+    if not isinstance(encoded, str):
+        encoded = encoded.encode()
+    h = L.fn2o_new_from_encoded(encoded.encode())
+    if not h:
+        raise ValueError("index-replay oracle cannot decode %r" % encoded)
+    try:
+        if kind == "positions":
+            pos = [np.ascontiguousarray(p, dtype=np.float32) for p in pos]
+            n, d = pos[0].size, len(pos)
+        else:
+            d = len(count) if kind == "grid" else 2
+            n = int(np.prod(count))
+        out = np.empty(max(n, 1), np.float32)
+        fp = np.zeros(max(n, 1), np.uint64)
+        mm = np.zeros(2, np.float32)
+        L.fn2o_set_trace(fp.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)))
+        if kind == "grid":
+            step = [1.0] * d if step is None else step
+            fn = {2: L.fn2o_gen_grid2d, 3: L.fn2o_gen_grid3d, 4: L.fn2o_gen_grid4d}[d]
+            rc = fn(h, _p(out), *[float(o) for o in offset], *[int(c) for c in count], *[float(s) for s in step],
+                    seed, mode, 1, 1, _p(mm))
+        elif kind == "positions":
+            off = [0.0] * d if offset is None else offset
+            fn = {2: L.fn2o_gen_pos2d, 3: L.fn2o_gen_pos3d, 4: L.fn2o_gen_pos4d}[d]
+            rc = fn(h, _p(out), n, *[_p(p) for p in pos], *[float(o) for o in off], seed, mode, 1, 1, _p(mm))
+        else:
+            step = [1.0, 1.0] if step is None else step
+            rc = L.fn2o_gen_tileable2d(h, _p(out), count[0], count[1], float(step[0]), float(step[1]), seed, mode, 1, 1, _p(mm))
+        L.fn2o_set_trace(None)
+        if rc:
+            raise ValueError("index-replay oracle rejected the call")
+        return out[:n], fp[:n]
+    finally:
+        L.fn2o_delete(h)

@@ -315,7 +315,7 @@ namespace
     int run( const DriverArgs& a, int mode, int lanes, int threads, float* minmax )
     {
         if( !a.root ) return -1;
-#ifndef FN2O_COUNTING
+#if !defined( FN2O_COUNTING ) && !defined( FN2O_TRACING )
         if( lanes == 16 )
         {
             __builtin_cpu_init();
@@ -537,7 +537,24 @@ void fn2o_reset_counters( void )
 }
 #endif
 
+#ifdef FN2O_TRACING
+// Index-replay build: the next generation calls of this thread write one
+// decision fingerprint per point into fingerprints (null stops it).
+void fn2o_set_trace( unsigned long long* fingerprints )
+{
+    fn2o_trace::out = reinterpret_cast<uint64_t*>( fingerprints );
+}
+#endif
+
 } // extern "C"
+
+#ifdef FN2O_TRACING
+namespace fn2o_trace
+{
+    thread_local uint64_t fp = 0;
+    thread_local uint64_t* out = nullptr;
+}
+#endif
 
 #ifdef FN2O_COUNTING
 namespace fn2o_count

@@ -42,6 +42,23 @@ namespace fn2o_count
 #define FN2O_CV() ( (void)0 )
 #endif
 
+#ifdef FN2O_TRACING
+// Index-replay build (one lane): every discrete decision the evaluation takes
+// (float -> int conversions, float comparisons, floor / round results, the
+// position bits White hashes) is folded into a per-point fingerprint, so that
+// two variants of the oracle agree on a point's fingerprint exactly when they
+// took the same lattice cells, simplex vertices, cells and selections there.
+namespace fn2o_trace
+{
+    extern thread_local uint64_t fp;
+    extern thread_local uint64_t* out;   // fingerprint per point of the running call, or null
+    inline void fold( uint32_t v ) { fp = ( fp ^ v ) * 0x100000001B3ull; }
+}
+#define FN2O_TRACE_I( iv ) do { for( int k_ = 0; k_ < N; k_++ ) fn2o_trace::fold( (uint32_t)( iv ).v[k_] ); } while( 0 )
+#else
+#define FN2O_TRACE_I( iv ) ( (void)0 )
+#endif
+
 #ifndef FN2O_INLINE
 #define FN2O_INLINE inline __attribute__( ( always_inline ) )
 #endif
@@ -121,6 +138,9 @@ namespace FN2O_NS
 #endif
         FV<N> r;
         FN2O_LOOP r.v[k] = floorf( a.v[k] );
+#ifdef FN2O_TRACING
+        FN2O_LOOP fn2o_trace::fold( bits_of( r.v[k] ) );
+#endif
         return r;
     }
     template<int N> FN2O_INLINE FV<N> vround( FV<N> a )
@@ -131,6 +151,9 @@ namespace FN2O_NS
 #endif
         FV<N> r;
         FN2O_LOOP r.v[k] = rintf( a.v[k] );
+#ifdef FN2O_TRACING
+        FN2O_LOOP fn2o_trace::fold( bits_of( r.v[k] ) );
+#endif
         return r;
     }
     template<int N> FN2O_INLINE FV<N> vabs( FV<N> a ) { FN2O_CV(); return mkf<N>( (typename FV<N>::T)( (typename IV<N>::T)a.v & 0x7fffffff ) ); }
@@ -181,6 +204,7 @@ namespace FN2O_NS
 #endif
         IV<N> r;
         FN2O_LOOP r.v[k] = cvti1( a.v[k] );
+        FN2O_TRACE_I( r );
         return r;
     }
     // float xor float (sign manipulation), float | float
@@ -210,11 +234,11 @@ namespace FN2O_NS
     template<int N> FN2O_INLINE IV<N> srl( IV<N> a, int s ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( uv( a ) >> s ) ); }
 
     // ---- masks: int32 lanes holding 0 or -1
-    template<int N> FN2O_INLINE IV<N> mlt( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v < b.v ) ); }
-    template<int N> FN2O_INLINE IV<N> mgt( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v > b.v ) ); }
-    template<int N> FN2O_INLINE IV<N> mge( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v >= b.v ) ); }
-    template<int N> FN2O_INLINE IV<N> meq( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v == b.v ) ); }
-    template<int N> FN2O_INLINE IV<N> mne( FV<N> a, FV<N> b ) { FN2O_CV(); return mki<N>( (typename IV<N>::T)( a.v != b.v ) ); }
+    template<int N> FN2O_INLINE IV<N> mlt( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r = mki<N>( (typename IV<N>::T)( a.v < b.v ) ); FN2O_TRACE_I( r ); return r; }
+    template<int N> FN2O_INLINE IV<N> mgt( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r = mki<N>( (typename IV<N>::T)( a.v > b.v ) ); FN2O_TRACE_I( r ); return r; }
+    template<int N> FN2O_INLINE IV<N> mge( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r = mki<N>( (typename IV<N>::T)( a.v >= b.v ) ); FN2O_TRACE_I( r ); return r; }
+    template<int N> FN2O_INLINE IV<N> meq( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r = mki<N>( (typename IV<N>::T)( a.v == b.v ) ); FN2O_TRACE_I( r ); return r; }
+    template<int N> FN2O_INLINE IV<N> mne( FV<N> a, FV<N> b ) { FN2O_CV(); IV<N> r = mki<N>( (typename IV<N>::T)( a.v != b.v ) ); FN2O_TRACE_I( r ); return r; }
     template<int N> FN2O_INLINE IV<N> mlt( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v < b.v ); }
     template<int N> FN2O_INLINE IV<N> mgt( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v > b.v ); }
     template<int N> FN2O_INLINE IV<N> mge( IV<N> a, IV<N> b ) { FN2O_CV(); return mki<N>( a.v >= b.v ); }

@@ -61,7 +61,13 @@ namespace FN2O_NS
                     // FMulAdd( Convert<float>( idx ), step, offset ) (Generator.inl:104-105, 160-162, 225-228)
                     for( int d = 0; d < D; d++ ) p.c[d] = E::fmadd( p.c[d], E::S( a.step[d] ), E::S( a.offset[d] ) );
                 }
+#ifdef FN2O_TRACING
+                fn2o_trace::fp = 0xCBF29CE484222325ull;
+#endif
                 F v = E::template gen<D>( a.root, E::SI( a.seed ), p );
+#ifdef FN2O_TRACING
+                if( fn2o_trace::out ) fn2o_trace::out[base] = fn2o_trace::fp;
+#endif
                 for( int k = 0; k < N && base + k < end; k++ )
                 {
                     a.out[base + k] = v.v[k];
@@ -93,7 +99,13 @@ namespace FN2O_NS
                     p.c[2].v[k] = sinf( xF ) * xFreq;
                     p.c[3].v[k] = sinf( yF ) * yFreq;
                 }
+#ifdef FN2O_TRACING
+                fn2o_trace::fp = 0xCBF29CE484222325ull;
+#endif
                 F v = E::template gen<4>( a.root, E::SI( a.seed ), p );
+#ifdef FN2O_TRACING
+                if( fn2o_trace::out ) fn2o_trace::out[base] = fn2o_trace::fp;
+#endif
                 for( int k = 0; k < N && base + k < end; k++ )
                 {
                     a.out[base + k] = v.v[k];

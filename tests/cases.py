@@ -147,3 +147,49 @@ def _cases():
 
 
 CASES = _cases()
+
+
+def node_count(spec, seen=None):
+    """Distinct nodes of a tree (a node used twice counts once)."""
+    from fastnoise2_amd.tree import NodeSpec
+    seen = set() if seen is None else seen
+    if id(spec) in seen:
+        return 0
+    seen.add(id(spec))
+    kids = list(spec.lookups.values()) + [h for h in spec.hybrids.values() if isinstance(h, NodeSpec)]
+    return 1 + sum(node_count(k, seen) for k in kids)
+
+
+def replay(spec, dims, seed, tol, values, mode):
+    """Index replay of one case's grid against an oracle variant (unfused or
+    AVX2 order): `values` (the fused result, from the GPU or the oracle) is
+    compared with the variant on the points whose decision fingerprints
+    (oracle.index_trace) agree with the fused variant's. Returns (flips,
+    ill-conditioned points, max error over the rest, its bound).
+
+    The bound is 1e-5 for a single node and 1e-4 for a graph (or the case's
+    own tolerance when larger), per point relative to max(1, |value|) for
+    outputs not confined to [-1, 1]; the returned error is that ratio times
+    the bound's base. A Divide's quotient is ill-conditioned where its
+    denominator is below 1e-2 in magnitude (an input's last-bit difference
+    then grows past 1e-5 of the quotient); those points are counted, not
+    bounded."""
+    import numpy as np
+    import oracle
+    r = REGION[dims]
+    enc = spec.encode()
+    fused, fp0 = oracle.index_trace(enc, "grid", oracle.FUSED, seed, r["offset"], r["count"], r["step"])
+    alt, fp1 = oracle.index_trace(enc, "grid", mode, seed, r["offset"], r["count"], r["step"])
+    agree = fp0 == fp1
+    ill = np.zeros(agree.shape, bool)
+    if spec.meta.name == "Divide":
+        den = spec.hybrids.get(1, 1.0)
+        if hasattr(den, "encode"):
+            dv, _ = oracle.index_trace(den.encode(), "grid", oracle.FUSED, seed, r["offset"], r["count"], r["step"])
+            ill = np.abs(dv) < 1e-2
+    values = np.asarray(values, np.float64)
+    ok = agree & ~ill & np.isfinite(alt) & np.isfinite(values)
+    bound = max(tol, 1e-5 if node_count(spec) == 1 else 1e-4)
+    rel = np.abs(values[ok] - alt[ok]) / np.maximum(1.0, np.abs(alt[ok].astype(np.float64)))
+    err = float(rel.max(initial=0.0))
+    return int((~agree).sum()), int((ill & agree).sum()), err, bound
