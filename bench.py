@@ -12,6 +12,12 @@ figures from HIP events on its launch stream, and the host baseline: the CPU
 oracle (a restatement of the reference algorithm, vectorised to 16 lanes and
 threaded) timed on a bounded sample of the same workload.
 
+The default run also measures the other plain workloads the same way under
+"configs" (Perlin 64^3, the tileable 1024^2 FBm image, the warped cellular
+256^3 grid and 16.7 M positions, and two cheap single-node 16384^2 grids that
+HBM writes bound), each with its roofline, hardware counters and host
+baseline, then the serving engine and the chunked-world client.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 import argparse
@@ -32,6 +38,21 @@ FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 vector peak (an FMA co
 VALU_PEAK_TOPS = FP32_PEAK_TFLOPS / 2
 HBM_PEAK_GBS = 8000.0
 COPY_CAL_MIB = 256                # device-to-device copy that scales the HBM counters
+SIMDS = 1024                      # 256 CUs x 4 SIMDs
+XCDS = 8                          # GRBM_GUI_ACTIVE reads as the sum over the 8 accelerator dies
+VALU_ISSUE_CYCLES = 2             # cycles per wave64 VALU instruction at the 78.6 T peak
+
+HEADLINE = "fbm_simplex_2d_grid_4096x4096"
+# the other plain workloads, measured under "configs" by the default run
+CONFIGS = ["perlin_3d_grid_64x64x64", "fbm_simplex_tileable_1024x1024", "warped_cellular_3d_grid_256x256x256",
+           "warped_cellular_positions_16777216", "constant_2d_grid_16384x16384", "checkerboard_2d_grid_16384x16384"]
+# workloads whose kernel writes more than it computes: priced against HBM
+HBM_BOUND = {"constant_2d_grid_16384x16384", "checkerboard_2d_grid_16384x16384"}
+# counter passes: one for FETCH_SIZE (3 TCC counters), one for WRITE_SIZE
+# (2 TCC) with the instruction and busy counters beside it
+SQ_COUNTERS = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES",
+               "SQ_INSTS_VALU_TRANS_F32", "GRBM_GUI_ACTIVE"]
+COUNTER_PASSES = [["FETCH_SIZE"], ["WRITE_SIZE"] + SQ_COUNTERS]
 
 
 def make_spec(fn, W, wl, keep):
@@ -46,12 +67,12 @@ def make_spec(fn, W, wl, keep):
 
 
 def cpu_baseline(wl, enc, target_s=12.0):
-    """Times the oracle (16 lanes, one thread per core of this process's CPU
-    share, at most 16) on a bounded sample of the workload: a slice of it when
+    """Times the oracle (16 lanes, one thread per core in this process's CPU
+    affinity mask) on a bounded sample of the workload: a slice of it when
     the whole takes longer than target_s, else whole passes repeated for about
     target_s."""
     import oracle
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = max(1, len(os.sched_getaffinity(0)))
     t = oracle.Tree(enc)
 
     def run(rows, reps=1):
@@ -102,48 +123,63 @@ class _RowView:
         return [a[sl] for a in self.arrays]
 
 
-def pmc_child(fn, W, wl, name):
+def pmc_child(fn, W, names):
     """Counter pass body: three 256 MiB device-to-device copies (the scale
-    reference), then the workload's kernel three times; prints its name."""
+    reference), then each workload's kernel four times (one launch that
+    compiles the tree's specialised kernel, three more); prints one JSON line
+    per workload naming its kernel."""
     nbytes = COPY_CAL_MIB << 20
     src, dst = fn.DeviceArray(nbytes // 4), fn.DeviceArray(nbytes // 4)
     for _ in range(3):
         if fn.lib().fn2Memcpy(dst.ptr, src.ptr, nbytes, 2):
             raise SystemExit("copy failed: " + fn.last_error())
-    node = fn.Node.from_encoded(wl["tree"]().encode())
-    keep = []
-    spec = make_spec(fn, W, wl, keep)
-    out = fn.DeviceArray(int(np.prod(wl["count"])))
-    fn.set_jit_mode(fn.JIT_SYNC)
-    node.bench_device(spec, out, warmup=0, iters=1)
-    fn.set_jit_mode(fn.JIT_ASYNC)
-    node.bench_device(spec, out, warmup=0, iters=3)
-    dims = wl["dims"] if wl["kind"] != "tileable" else 4
-    print(json.dumps({"kernel": "fn2_jit_single" if node.jit_state(dims) == 1 else "fn2_generate"}), flush=True)
+    src.free()
+    dst.free()
+    wls = W.bench_workloads()
+    for name in names:
+        wl = wls[name]
+        node = fn.Node.from_encoded(wl["tree"]().encode())
+        keep = []
+        spec = make_spec(fn, W, wl, keep)
+        out = fn.DeviceArray(int(np.prod(wl["count"])))
+        fn.set_jit_mode(fn.JIT_SYNC)
+        node.bench_device(spec, out, warmup=0, iters=1)
+        fn.set_jit_mode(fn.JIT_ASYNC)
+        node.bench_device(spec, out, warmup=0, iters=3)
+        dims = wl["dims"] if wl["kind"] != "tileable" else 4
+        fn.lib().fn2DeviceSynchronize()
+        out.free()
+        for d in keep:
+            d.free()
+        print(json.dumps({"workload": name, "kernel": "fn2_jit_single" if node.jit_state(dims) == 1 else "fn2_generate",
+                          "dispatches": 4}), flush=True)
 
 
-def measure_traffic(workload):
-    """HBM bytes per launch of the workload's kernel: one rocprofv3 --pmc pass
-    for FETCH_SIZE, one for WRITE_SIZE (KiB per dispatch), each scaled by what
-    the same counter reads for the 256 MiB copy (which moves 256 MiB each way).
-    None when rocprofv3 is missing or a pass fails."""
+def _is_generator(kernel_name):
+    return kernel_name.startswith("fn2_jit_single") or kernel_name.startswith("void fn2_generate<")
+
+
+def measure_counters(names):
+    """Hardware counters per launch of each workload's kernel, from two
+    rocprofv3 --pmc passes over one child run of all the workloads: HBM bytes
+    (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch, each scaled by what the
+    same counter reads for the 256 MiB copy, which moves 256 MiB each way) and
+    the instruction / busy counters. {} when rocprofv3 is missing or a pass
+    fails."""
     import csv
     import shutil
     import subprocess
     import tempfile
     if not shutil.which("rocprofv3"):
-        return None
-    res = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        return {}
+    res = {n: {} for n in names}
+    for counters in COUNTER_PASSES:
         d = tempfile.mkdtemp(prefix="fn2_pmc_")
-        cmd = ["timeout", "-s", "KILL", "120", "rocprofv3", "--pmc", counter, "--kernel-trace", "-f", "csv",
-               "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--workload", workload,
-               "--pmc-child"]
+        cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc"] + counters + [
+            "--kernel-trace", "-f", "csv", "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__),
+            "--pmc-child", ",".join(names)]
         p = subprocess.run(cmd, capture_output=True, text=True, env=dict(os.environ, FN2_JIT="async"))
-        kernel = None
-        for line in p.stdout.splitlines():
-            if line.startswith("{"):
-                kernel = json.loads(line).get("kernel")
+        order = [json.loads(line) for line in p.stdout.splitlines() if line.startswith("{")]
         rows = []
         for root, _, files in os.walk(d):
             for f in files:
@@ -151,18 +187,91 @@ def measure_traffic(workload):
                     with open(os.path.join(root, f)) as fh:
                         rows += list(csv.DictReader(fh))
         shutil.rmtree(d, ignore_errors=True)
-        copy = [float(r["Counter_Value"]) for r in rows
-                if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer") and int(r["Grid_Size"]) >= 1 << 16]
-        gen = [float(r["Counter_Value"]) for r in rows if kernel and kernel in r["Kernel_Name"]]
-        if not copy or len(gen) < 2:
-            progress("%s pass gave no usable rows (exit %d, kernel %r, %d rows); its output ends with: %s"
-                     % (counter, p.returncode, kernel, len(rows), (p.stdout + p.stderr)[-1500:]))
-            return None
-        scale = (COPY_CAL_MIB << 10) / (sum(copy) / len(copy))
-        kib = sum(gen[1:]) / len(gen[1:])   # the first dispatch includes compile-time warm-up
-        res[counter] = {"kib_per_dispatch": round(kib, 1), "copy_scale": round(scale, 3)}
-    total = sum(v["kib_per_dispatch"] * v["copy_scale"] * 1024 for v in res.values())
-    return {"bytes": int(total), "kernel": kernel, **res}
+        # counter value per (dispatch, counter), dispatches in launch order
+        per = {}
+        for r in rows:
+            key = int(r["Dispatch_Id"])
+            e = per.setdefault(key, {"kernel": r["Kernel_Name"], "grid": int(r["Grid_Size"]), "c": {}})
+            e["c"][r["Counter_Name"]] = e["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        disp = [per[k] for k in sorted(per)]
+        copies = [e for e in disp if e["kernel"].startswith("__amd_rocclr_copyBuffer") and e["grid"] >= 1 << 16]
+        gens = [e for e in disp if _is_generator(e["kernel"])]
+        if len(order) != len(names) or not copies or len(gens) != sum(o["dispatches"] for o in order):
+            progress("counter pass %s gave no usable rows (exit %d, %d workloads, %d generator dispatches); its output "
+                     "ends with: %s" % (counters[0], p.returncode, len(order), len(gens), (p.stdout + p.stderr)[-1500:]))
+            return {}
+        at = 0
+        for o in order:
+            mine = gens[at:at + o["dispatches"]][1:]   # the first dispatch includes compile-time warm-up
+            at += o["dispatches"]
+            rec = res[o["workload"]]
+            rec["kernel"] = o["kernel"]
+            for c in counters:
+                v = sum(e["c"].get(c, 0.0) for e in mine) / len(mine)
+                if c in ("FETCH_SIZE", "WRITE_SIZE"):
+                    scale = (COPY_CAL_MIB << 10) / (sum(e["c"][c] for e in copies) / len(copies))
+                    rec[c] = {"kib_per_dispatch": round(v, 1), "copy_scale": round(scale, 3)}
+                else:
+                    rec.setdefault("counters", {})[c] = round(v)
+    for rec in res.values():
+        if "FETCH_SIZE" in rec and "WRITE_SIZE" in rec:
+            rec["bytes"] = int(sum(rec[c]["kib_per_dispatch"] * rec[c]["copy_scale"] * 1024 for c in ("FETCH_SIZE", "WRITE_SIZE")))
+    return res
+
+
+def roofline_for(name, wl, n, kernel_ms, counters, kernel_name, interp_ms=None):
+    """Roofline figures of one workload's kernel: algorithmic work per point
+    (counted by the op-counting oracle, committed with the workload) over the
+    event-timed launch, against the VALU issue peak, or against HBM for the
+    workloads bound by their writes; plus what the counters show the
+    hardware issued and how busy its vector ALUs were."""
+    flop_pt, valu_pt = wl["ops"]
+    kernel_s = kernel_ms * 1e-3
+    bytes_pt = 4 + (4 * wl["dims"] if wl["kind"] == "positions" else 0)
+    valu_tops = valu_pt * n / kernel_s / 1e12
+    hbm_gbs = bytes_pt * n / kernel_s / 1e9
+    if name in HBM_BOUND:
+        r = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
+    else:
+        # vector-ALU bound: integer hashing and per-lane FP32, no contraction
+        # for MFMA and 4 bytes of HBM traffic per point
+        r = {"bound": "valu", "achieved": round(valu_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
+             "unit": "T VALU lane-ops/s", "frac": round(valu_tops / VALU_PEAK_TOPS, 4)}
+    r.update({
+        "traffic": counters.get("bytes"),
+        "traffic_algorithmic_bytes": bytes_pt * n,
+        "kernel": kernel_name,
+        "kernel_ms": round(kernel_ms, 4),
+        "valu_ops_per_point": valu_pt,
+        "flop_per_point": flop_pt,
+        "fp32_TFLOPs": round(flop_pt * n / kernel_s / 1e12, 3),
+        "fp32_peak_TFLOPs": FP32_PEAK_TFLOPS,
+        "hbm_algorithmic_GBs": round(hbm_gbs, 1),
+        "hbm_peak_GBs": HBM_PEAK_GBS,
+    })
+    if interp_ms is not None:
+        r["interpreter_kernel_ms"] = round(interp_ms, 4)
+    if counters.get("FETCH_SIZE"):
+        r["traffic_detail"] = {c: counters[c] for c in ("FETCH_SIZE", "WRITE_SIZE") if c in counters}
+        if counters.get("WRITE_SIZE"):
+            w = counters["WRITE_SIZE"]
+            r["hbm_write_GBs_counted"] = round(w["kib_per_dispatch"] * w["copy_scale"] * 1024 / kernel_s / 1e9, 1)
+    c = counters.get("counters")
+    if c and c.get("SQ_INSTS_VALU"):
+        # SQ_INSTS_VALU counts wave64 instructions: 64 lane operations each
+        issued = c["SQ_INSTS_VALU"] * 64 / kernel_s / 1e12
+        cycles = c.get("GRBM_GUI_ACTIVE", 0) / XCDS
+        r["hw"] = {
+            "valu_instr_per_point": round(c["SQ_INSTS_VALU"] * 64 / n, 1),
+            "issued_T_lane_ops_s": round(issued, 3),
+            "issue_frac": round(issued / VALU_PEAK_TOPS, 4),
+            # share of the kernel's cycles each SIMD spent issuing VALU work
+            "valu_busy": round(c["SQ_INSTS_VALU"] * VALU_ISSUE_CYCLES / SIMDS / cycles, 4) if cycles else None,
+            "clock_GHz": round(cycles / kernel_s / 1e9, 3) if cycles else None,
+            "counters": c,
+        }
+    return r
 
 
 def progress(msg):
@@ -216,7 +325,7 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=20000)
     return out
 
 
-def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
+def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000, modes=("plain", "batched", "graph", "persistent")):
     """The mixed-tree serving workload: 8 seeded random trees (depth 3..12),
     2D/3D/4D requests of 2^12..2^22 points, bursty on/off arrivals (10 ms at
     10k req/s); per-tree tuned batched launches vs graph replay vs the
@@ -230,6 +339,8 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
            % trace_requests, "mean_points": round(float(pts.mean()))}
     warm = [W.grid_spec_like(d) for d in (2, 3, 4)]
     for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH), ("persistent", SERVE_PERSISTENT)):
+        if name not in modes:
+            continue
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tids = [eng.add_tree(e) for e in encs]
         wt = [t for t in tids for _ in warm]
@@ -249,7 +360,8 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000):
         eng.close()
         out[name] = rec
         progress("mixed serving %s done" % name)
-    out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
+    if "plain" in out and "batched" in out:
+        out["batched_vs_plain_throughput"] = round(out["batched"]["saturated_Gpoints_s"] / out["plain"]["saturated_Gpoints_s"], 2)
     return out
 
 
@@ -300,21 +412,105 @@ def world_bench(W, chunk_size=64, load_range=400.0, step=64.0, frames=48):
     return out
 
 
+def time_workload(fn, W, name, wl, steps, warmup, jit=True):
+    """Device-resident throughput of one workload: the interpreter's event time
+    per launch, then the tree's specialised kernel (compiled before timing)
+    over `steps` launches. Returns (Gpoints/s, event ms, interpreter ms,
+    kernel name, node)."""
+    node = fn.Node.from_encoded(wl["tree"]().encode())
+    keep = []
+    spec = make_spec(fn, W, wl, keep)
+    n = int(np.prod(wl["count"]))
+    out = fn.DeviceArray(n)
+    fn.set_jit_mode(fn.JIT_OFF)
+    interp_ms, _ = node.bench_device(spec, out, warmup=min(warmup, 2), iters=max(3, steps // 5))
+    kernel_name = "fn2_generate"
+    if jit:
+        fn.set_jit_mode(fn.JIT_SYNC)
+        node.bench_device(spec, out, warmup=0, iters=1)
+        if node.jit_state(wl["dims"] if wl["kind"] != "tileable" else 4) == 1:
+            kernel_name = "fn2_jit_single"
+    ms_event, ms_wall = node.bench_device(spec, out, warmup=warmup, iters=steps)
+    fn.set_jit_mode(fn.JIT_ASYNC)
+    out.free()
+    for d in keep:
+        d.free()
+    return n / (ms_wall * 1e-3) / 1e9, ms_event, interp_ms, kernel_name, node
+
+
+def api_roundtrip(fn, W, node, wl, reps=5):
+    """The reference's own call shape: GenUniformGrid / GenTileable2D /
+    GenPositionArray3D into a caller-owned pageable host buffer (allocated and
+    touched once, as a caller reuses it), host-to-device position copies and
+    the device-to-host output copy included. Gpoints/s over `reps` calls."""
+    import ctypes
+    n = int(np.prod(wl["count"]))
+    out = np.zeros(n, dtype=np.float32)
+    mm = np.zeros(2, dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    o, m = out.ctypes.data_as(fp), mm.ctypes.data_as(fp)
+    L = fn.lib()
+    if wl["kind"] == "grid":
+        off, cnt, seed = wl["offset"], wl["count"], wl["seed"]
+        if wl["dims"] == 2:
+            call = lambda: L.fnGenUniformGrid2D(node.handle, o, *map(float, off), *cnt, 1.0, 1.0, seed, m)
+        else:
+            call = lambda: L.fnGenUniformGrid3D(node.handle, o, *map(float, off), *cnt, 1.0, 1.0, 1.0, seed, m)
+    elif wl["kind"] == "tileable":
+        call = lambda: L.fnGenTileable2D(node.handle, o, wl["count"][0], wl["count"][1], 1.0, 1.0, wl["seed"], m)
+    else:
+        xyz = [np.ascontiguousarray(a) for a in W.splitmix64_positions(n, seed=7)]
+        px, py, pz = (a.ctypes.data_as(fp) for a in xyz)
+        call = lambda: L.fnGenPositionArray3D(node.handle, o, n, px, py, pz, 0.0, 0.0, 0.0, wl["seed"], m)
+    call()
+    if fn.last_error():
+        raise RuntimeError("API call failed: " + fn.last_error())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    return round(n * reps / (time.perf_counter() - t0) / 1e9, 3)
+
+
+def config_record(fn, W, name, wl, counters, args):
+    """One plain workload: throughput, roofline, counters, API round trip and
+    host baseline."""
+    n = int(np.prod(wl["count"]))
+    # enough launches for about 0.3 s of GPU time, at least 10
+    value, ms, interp_ms, kname, node = time_workload(fn, W, name, wl, 3, 1)
+    steps = int(min(2000, max(10, 0.3 / max(ms * 1e-3, 1e-6))))
+    value, ms, interp_ms, kname, node = time_workload(fn, W, name, wl, steps, 3)
+    rec = {"value": round(value, 3), "unit": "Gpoints/s", "points": n, "steps": steps, "kernel_ms": round(ms, 4),
+           "tree": wl["tree"]().encode(), "seed": wl["seed"],
+           "roofline": roofline_for(name, wl, n, ms, counters.get(name, {}), kname, interp_ms)}
+    rec["api_roundtrip_Gpoints_s"] = api_roundtrip(fn, W, node, wl, reps=5 if n >= 1 << 20 else 50)
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(wl, rec["tree"], args.config_cpu_seconds)
+        rec["gpu_vs_cpu"] = round(value / rec["cpu_baseline"]["value"], 1)
+    progress("config %s: %.2f Gpoints/s, kernel %.4f ms" % (name, value, ms))
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="fbm_simplex_2d_grid_4096x4096")
+    ap.add_argument("--workload", default=HEADLINE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--config-cpu-seconds", type=float, default=4.0,
+                    help="host baseline sample per workload of the configs section")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-mixed-only", default=None, metavar="MODES",
+                    help="only the mixed-tree serving replays, in the comma-separated modes "
+                         "(plain,batched,graph,persistent); for kernel-trace accounting")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other plain workloads")
     ap.add_argument("--jit", default="sync", choices=["off", "sync"],
                     help="run-time specialised kernel of the tree (compiled before timing) or the interpreter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (the reference kernel for HBM counter scaling)")
-    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes for roofline.traffic")
-    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a single MI355X; "
@@ -322,18 +518,23 @@ def main():
         sys.exit(2)
 
     import fastnoise2_amd.workloads as W
+    if args.serving_mixed_only:
+        print(json.dumps({"serving_mixed": serving_mixed_bench(W, modes=args.serving_mixed_only.split(","))}), flush=True)
+        return
     wls = W.bench_workloads()
+    with_configs = args.workload == HEADLINE and not args.no_configs
+    names = [args.workload] + (CONFIGS if with_configs else [])
     # counter passes first: they run in child processes, started before this
     # process touches the GPU
-    traffic = None
+    counters = {}
     if not args.no_traffic and not args.pmc_child and not args.copy_calibration:
-        traffic = measure_traffic(args.workload)
-        progress("hbm counter passes done")
+        counters = measure_counters(names)
+        progress("counter passes done")
     import fastnoise2_amd as fn
     if fn.device_count() < 1:
         raise SystemExit("no HIP device visible")
     if args.pmc_child:
-        pmc_child(fn, W, wls[args.workload], args.workload)
+        pmc_child(fn, W, args.pmc_child.split(","))
         return
     if args.copy_calibration:
         nbytes = args.copy_calibration << 20
@@ -344,58 +545,13 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
         return
     wl = wls[args.workload]
-    spec_tree = wl["tree"]()
-    enc = spec_tree.encode()
-    node = fn.Node.from_encoded(enc)
-    keep = []
-    spec = make_spec(fn, W, wl, keep)
+    enc = wl["tree"]().encode()
     n = int(np.prod(wl["count"]))
-    out = fn.DeviceArray(n)
-
-    # the interpreter's figure first, then the tree's specialised kernel
-    fn.set_jit_mode(fn.JIT_OFF)
-    interp_ms, _ = node.bench_device(spec, out, warmup=args.warmup, iters=max(3, args.steps // 5))
-    kernel_name = "fn2_generate"
-    if args.jit == "sync":
-        fn.set_jit_mode(fn.JIT_SYNC)
-        t0 = time.perf_counter()
-        node.bench_device(spec, out, warmup=0, iters=1)
-        jit_compile_s = time.perf_counter() - t0
-        if node.jit_state(wl["dims"] if wl["kind"] != "tileable" else 4) == 1:
-            kernel_name = "fn2_jit_single"
-    ms_event, ms_wall = node.bench_device(spec, out, warmup=args.warmup, iters=args.steps)
-    value = n / (ms_wall * 1e-3) / 1e9
-    fn.set_jit_mode(fn.JIT_ASYNC)
+    t0 = time.perf_counter()
+    value, ms_event, interp_ms, kernel_name, node = time_workload(fn, W, args.workload, wl, args.steps, args.warmup,
+                                                                   jit=args.jit == "sync")
+    ms_wall = n / (value * 1e9) * 1e3
     progress("%s: %d timed steps done, %.2f Gpoints/s" % (args.workload, args.steps, value))
-
-    # algorithmic work per point (counted by the op-counting oracle, committed
-    # with the workload and recounted by the CPU tests)
-    flop_pt, valu_pt = wl["ops"]
-    kernel_s = ms_event * 1e-3
-    tflops = flop_pt * n / kernel_s / 1e12
-    bytes_pt = 4 + (4 * wl["dims"] if wl["kind"] == "positions" else 0)
-    valu_tops = valu_pt * n / kernel_s / 1e12
-    roofline = {
-        # vector-ALU bound: integer hashing and per-lane FP32, no contraction
-        # for MFMA and 4 bytes of HBM traffic per point
-        "bound": "valu",
-        "achieved": round(valu_tops, 3),
-        "peak": round(VALU_PEAK_TOPS, 2),
-        "unit": "T VALU lane-ops/s",
-        "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
-        "traffic": traffic["bytes"] if traffic else None,
-        "traffic_detail": traffic,
-        "traffic_algorithmic_bytes": bytes_pt * n,
-        "kernel": kernel_name,
-        "interpreter_kernel_ms": round(interp_ms, 4),
-        "kernel_ms": round(ms_event, 4),
-        "valu_ops_per_point": valu_pt,
-        "flop_per_point": flop_pt,
-        "fp32_TFLOPs": round(tflops, 3),
-        "fp32_peak_TFLOPs": FP32_PEAK_TFLOPS,
-        "hbm_algorithmic_GBs": round(bytes_pt * n / kernel_s / 1e9, 1),
-        "hbm_peak_GBs": HBM_PEAK_GBS,
-    }
     rec = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -410,24 +566,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic: grid coordinates / SplitMix64(7) positions, output device-resident",
         "config": {"workload": args.workload, "tree": enc, "points": n, "seed": wl["seed"]},
-        "jit_compile_s": round(jit_compile_s, 2) if args.jit == "sync" else None,
-        "roofline": roofline,
+        "roofline": roofline_for(args.workload, wl, n, ms_event, counters.get(args.workload, {}), kernel_name, interp_ms),
     }
-    if wl["kind"] == "grid":
-        # the reference's own call shape: host output buffer, PCIe copy included
-        d = wl["dims"]
-        gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d}[d]
-        gen(*wl["offset"], *wl["count"], seed=wl["seed"])
-        reps = 5
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            gen(*wl["offset"], *wl["count"], seed=wl["seed"])
-        rec["api_roundtrip_Gpoints_s"] = round(n * reps / (time.perf_counter() - t0) / 1e9, 3)
+    rec["api_roundtrip_Gpoints_s"] = api_roundtrip(fn, W, node, wl)
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(wl, enc, args.cpu_seconds)
         rec["gpu_vs_cpu"] = round(value / rec["cpu_baseline"]["value"], 1)
         progress("cpu baseline done")
-    if not args.no_serving and args.workload == "fbm_simplex_2d_grid_4096x4096":
+    if with_configs:
+        rec["configs"] = {name: config_record(fn, W, name, wls[name], counters, args) for name in CONFIGS}
+    if not args.no_serving and args.workload == HEADLINE:
         rec["serving"] = serving_bench(W)
         rec["serving_mixed"] = serving_mixed_bench(W)
         rec["world"] = world_bench(W)

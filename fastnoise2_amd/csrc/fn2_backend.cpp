@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "fn2_backend.h"
+#include "fn2_copy_pool.h"
 #include "fn2_internal.h"
 #include "fn2_jit.h"
 #include "fn2_gpu.h"
@@ -187,6 +188,48 @@ namespace fn2
         }
     }
 
+    bool ValidateProgram( const Program& p, int& regs, std::string& err )
+    {
+        // every register an instruction touches (vector operands span dim
+        // registers) must lie below the program's register count, itself at
+        // most the register file; every constant-pool window (at most 16
+        // floats) must lie inside the uploaded pool
+        regs = std::max( 1, std::max( p.regsUsed, p.dims ) );
+        if( regs > kMaxRegs )
+        {
+            err = "compiled program uses more registers than the register file holds";
+            return false;
+        }
+        for( const Ins& in : p.ins )
+        {
+            if( in.op == OP_END || in.op >= OP_COUNT || in.dim > 4 )
+            {
+                err = "compiled program has an invalid instruction";
+                return false;
+            }
+            for( int r = 0; r < 6; r++ )
+            {
+                const int ext = OperandExtent( in, r );
+                if( in.r[r] != kNoReg && ext > 0 && in.r[r] + ext > regs )
+                {
+                    err = "compiled program addresses a register outside its register file";
+                    return false;
+                }
+            }
+            if( in.k > p.konst.size() )
+            {
+                err = "compiled program addresses constants outside its pool";
+                return false;
+            }
+        }
+        if( p.resultReg < 0 || p.resultReg >= kMaxRegs || p.ins.size() > (size_t)kMaxInstructions )
+        {
+            err = "compiled program is malformed";
+            return false;
+        }
+        return true;
+    }
+
     const DeviceProgram* GetDeviceProgram( const Program& p )
     {
         std::lock_guard<std::mutex> lk( gMu );
@@ -205,41 +248,11 @@ namespace fn2
             SetError( "device program cache full (more than 8192 distinct compiled trees)" );
             return nullptr;
         }
-        // every register an instruction touches (vector operands span dim
-        // registers) must lie below the program's register count, itself at
-        // most the register file; every constant-pool window (at most 16
-        // floats) must lie inside the uploaded pool
-        const int regs = std::max( 1, std::max( p.regsUsed, p.dims ) );
-        if( regs > kMaxRegs )
+        int regs = 1;
+        std::string err;
+        if( !ValidateProgram( p, regs, err ) )
         {
-            SetError( "compiled program uses more registers than the register file holds" );
-            return nullptr;
-        }
-        for( const Ins& in : p.ins )
-        {
-            if( in.op == OP_END || in.op >= OP_COUNT || in.dim > 4 )
-            {
-                SetError( "compiled program has an invalid instruction" );
-                return nullptr;
-            }
-            for( int r = 0; r < 6; r++ )
-            {
-                const int ext = OperandExtent( in, r );
-                if( in.r[r] != kNoReg && ext > 0 && in.r[r] + ext > regs )
-                {
-                    SetError( "compiled program addresses a register outside its register file" );
-                    return nullptr;
-                }
-            }
-            if( in.k > p.konst.size() )
-            {
-                SetError( "compiled program addresses constants outside its pool" );
-                return nullptr;
-            }
-        }
-        if( p.resultReg < 0 || p.resultReg >= kMaxRegs || p.ins.size() > (size_t)kMaxInstructions )
-        {
-            SetError( "compiled program is malformed" );
+            SetError( err );
             return nullptr;
         }
         DeviceProgram* d = new DeviceProgram;
@@ -327,6 +340,80 @@ namespace fn2
         return true;
     }
 
+    // Host copies of the staged entry points: one pool for the process, used
+    // by one call at a time (the copies are bound by memory bandwidth).
+    std::mutex gHostCopyMu;
+
+    CopyPool& HostCopies()
+    {
+        static CopyPool* pool = new CopyPool( 3 );   // three workers plus the caller
+        return *pool;
+    }
+
+    void HostCopy( float* dst, const float* src, int64_t count )
+    {
+        std::lock_guard<std::mutex> lk( gHostCopyMu );
+        HostCopies().Run( { { dst, src, (size_t)count } } );
+    }
+
+    bool CallContext::ReserveStage()
+    {
+        if( hStage ) return true;
+        for( int i = 0; i < kStageSlots; i++ )
+        {
+            if( !evStage[i] && !HipOk( hipEventCreateWithFlags( &evStage[i], hipEventDisableTiming ), "hipEventCreate" ) ) return false;
+        }
+        return HipOk( hipHostMalloc( (void**)&hStage, (size_t)kStageSlots * kStageChunkFloats * sizeof( float ), hipHostMallocDefault ),
+                      "hipHostMalloc(staging ring)" );
+    }
+
+    bool CallContext::StagedToHost( float* dst, const float* dSrc, int64_t count )
+    {
+        if( !ReserveStage() ) return false;
+        const int64_t chunks = ( count + kStageChunkFloats - 1 ) / kStageChunkFloats;
+        auto issue = [&]( int64_t k ) {
+            const int64_t b = k * kStageChunkFloats, m = std::min( kStageChunkFloats, count - b );
+            float* slot = hStage + ( k % kStageSlots ) * kStageChunkFloats;
+            return HipOk( hipMemcpyAsync( slot, dSrc + b, (size_t)m * sizeof( float ), hipMemcpyDeviceToHost, stream ), "copy output" ) &&
+                   HipOk( hipEventRecord( evStage[k % kStageSlots], stream ), "hipEventRecord" );
+        };
+        for( int64_t k = 0; k < std::min<int64_t>( chunks, kStageSlots ); k++ )
+        {
+            if( !issue( k ) ) return false;
+        }
+        for( int64_t k = 0; k < chunks; k++ )
+        {
+            if( !HipOk( hipEventSynchronize( evStage[k % kStageSlots] ), "generator kernel" ) ) return false;
+            const int64_t b = k * kStageChunkFloats, m = std::min( kStageChunkFloats, count - b );
+            HostCopy( dst + b, hStage + ( k % kStageSlots ) * kStageChunkFloats, m );
+            // the slot is free again: the chunk kStageSlots ahead goes into it
+            if( k + kStageSlots < chunks && !issue( k + kStageSlots ) ) return false;
+        }
+        return true;
+    }
+
+    bool CallContext::StagedToDevice( float* dDst, const float* src, int64_t count )
+    {
+        if( !ReserveStage() ) return false;
+        const int64_t chunks = ( count + kStageChunkFloats - 1 ) / kStageChunkFloats;
+        for( int64_t k = 0; k < chunks; k++ )
+        {
+            const int s = (int)( k % kStageSlots );
+            // the slot's previous upload (of this call, or of the axis before)
+            // must have left it; an event never recorded is complete
+            if( !HipOk( hipEventSynchronize( evStage[s] ), "copy positions" ) ) return false;
+            const int64_t b = k * kStageChunkFloats, m = std::min( kStageChunkFloats, count - b );
+            float* slot = hStage + (int64_t)s * kStageChunkFloats;
+            HostCopy( slot, src + b, m );
+            if( !HipOk( hipMemcpyAsync( dDst + b, slot, (size_t)m * sizeof( float ), hipMemcpyHostToDevice, stream ), "copy positions" ) ||
+                !HipOk( hipEventRecord( evStage[s], stream ), "hipEventRecord" ) )
+            {
+                return false;
+            }
+        }
+        return true;
+    }
+
     bool IsDevicePointer( const void* p )
     {
         if( !p ) return false;
@@ -377,6 +464,29 @@ namespace fn2
         return n;
     }
 
+    void FillTileTable( int xSize, int ySize, float xStep, float yStep, float* t )
+    {
+        const float pi2Recip = 0.15915493667f;
+        const float xSizePi = (float)xSize * pi2Recip;
+        const float ySizePi = (float)ySize * pi2Recip;
+        const float xFreq = xSizePi * xStep;
+        const float yFreq = ySizePi * yStep;
+        const float xMul = 1 / xSizePi;
+        const float yMul = 1 / ySizePi;
+        for( int i = 0; i < xSize; i++ )
+        {
+            float f = (float)i * xMul;
+            t[i] = cosf( f ) * xFreq;
+            t[xSize + i] = sinf( f ) * xFreq;
+        }
+        for( int j = 0; j < ySize; j++ )
+        {
+            float f = (float)j * yMul;
+            t[2 * xSize + j] = cosf( f ) * yFreq;
+            t[2 * xSize + ySize + j] = sinf( f ) * yFreq;
+        }
+    }
+
     // GenTileable2D maps column x and row y onto a 4D torus (Generator.inl:370-433
     // of the reference): x' = cos( x * xMul ) * xFreq, z' = sin( x * xMul ) * xFreq
     // and the same for y with w'. Those coordinates depend on the column or the
@@ -396,26 +506,8 @@ namespace fn2
             err = "more than 1024 distinct tileable sizes in one process";
             return nullptr;
         }
-        const float pi2Recip = 0.15915493667f;
-        const float xSizePi = (float)xSize * pi2Recip;
-        const float ySizePi = (float)ySize * pi2Recip;
-        const float xFreq = xSizePi * xStep;
-        const float yFreq = ySizePi * yStep;
-        const float xMul = 1 / xSizePi;
-        const float yMul = 1 / ySizePi;
         std::vector<float> t( 2 * ( (size_t)xSize + ySize ) );
-        for( int i = 0; i < xSize; i++ )
-        {
-            float f = (float)i * xMul;
-            t[i] = cosf( f ) * xFreq;
-            t[xSize + i] = sinf( f ) * xFreq;
-        }
-        for( int j = 0; j < ySize; j++ )
-        {
-            float f = (float)j * yMul;
-            t[2 * xSize + j] = cosf( f ) * yFreq;
-            t[2 * xSize + ySize + j] = sinf( f ) * yFreq;
-        }
+        FillTileTable( xSize, ySize, xStep, yStep, t.data() );
         float* d = nullptr;
         hipError_t e = hipMalloc( (void**)&d, t.size() * sizeof( float ) );
         if( e == hipSuccess ) e = UploadSync( d, t.data(), t.size() * sizeof( float ) );
@@ -428,7 +520,7 @@ namespace fn2
         return d;
     }
 
-    bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err )
+    bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err, const float* tileTable )
     {
         a.rangeStride = kRangeStrideSpread;   // single calls; the serving engine packs its records
         a.kind = s.kind;
@@ -444,7 +536,7 @@ namespace fn2
             if( total > 0 )
             {
                 a.div[0] = MakeFastDiv( (uint32_t)s.count[0] );
-                const float* tab = TileTable( s.count[0], s.count[1], s.step[0], s.step[1], err );
+                const float* tab = tileTable ? tileTable : TileTable( s.count[0], s.count[1], s.step[0], s.step[1], err );
                 if( !tab ) return false;
                 a.pos[0] = tab;
                 a.pos[1] = tab + 2 * s.count[0];
@@ -557,7 +649,14 @@ namespace fn2
                 {
                     if( !hostPos[d] ) return fail( "null position array" );
                     float* dst = c->dPos + (size_t)d * total;
-                    if( !HipOk( hipMemcpyAsync( dst, hostPos[d], (size_t)total * sizeof( float ), hipMemcpyDefault, c->stream ), "copy positions" ) ) return fail( tLastError );
+                    if( total >= kStagedMinFloats && !IsDevicePointer( hostPos[d] ) )
+                    {
+                        if( !c->StagedToDevice( dst, hostPos[d], total ) ) return fail( tLastError );
+                    }
+                    else if( !HipOk( hipMemcpyAsync( dst, hostPos[d], (size_t)total * sizeof( float ), hipMemcpyDefault, c->stream ), "copy positions" ) )
+                    {
+                        return fail( tLastError );
+                    }
                     a.pos[d] = dst;
                 }
             }
@@ -568,7 +667,14 @@ namespace fn2
         if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, kCallRangeInts * sizeof( int32_t ), hipMemcpyDeviceToHost, c->stream ), "read min/max" ) ) return fail( tLastError );
         if( !devOut )
         {
-            if( !HipOk( hipMemcpyAsync( out, a.out, (size_t)total * sizeof( float ), hipMemcpyDeviceToHost, c->stream ), "copy output" ) ) return fail( tLastError );
+            if( total >= kStagedMinFloats )
+            {
+                if( !c->StagedToHost( out, a.out, total ) ) return fail( tLastError );
+            }
+            else if( !HipOk( hipMemcpyAsync( out, a.out, (size_t)total * sizeof( float ), hipMemcpyDeviceToHost, c->stream ), "copy output" ) )
+            {
+                return fail( tLastError );
+            }
         }
         if( !HipOk( hipStreamSynchronize( c->stream ), "generator kernel" ) ) return fail( tLastError );
 

@@ -26,6 +26,14 @@ namespace fn2
         float* dKonst = nullptr;
     };
 
+    // Host buffers of at least this many floats move through the pinned
+    // staging ring of a call context in chunks: the DMA of one chunk overlaps
+    // the multi-threaded copy of the previous one between pinned and pageable
+    // memory (a single pageable hipMemcpy runs at about 10 GB/s).
+    constexpr int kStageSlots = 3;
+    constexpr int64_t kStageChunkFloats = 1 << 21;   // 8 MiB
+    constexpr int64_t kStagedMinFloats = 1 << 18;    // 1 MiB
+
     // Per-call stream and staging buffers, pooled.
     struct CallContext
     {
@@ -37,8 +45,15 @@ namespace fn2
         int32_t* dKeys = nullptr;   // one range record, slots kRangeStrideSpread ints apart
         int32_t* hKeys = nullptr;   // pinned, the same size
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        float* hStage = nullptr;    // pinned ring: kStageSlots chunks of kStageChunkFloats
+        hipEvent_t evStage[kStageSlots] = {};
 
         bool Reserve( float*& buf, size_t& cap, size_t floats );
+        bool ReserveStage();
+        // device -> pageable host and pageable host -> device through the ring,
+        // on the context's stream; both return when the host side is done
+        bool StagedToHost( float* dst, const float* dSrc, int64_t count );
+        bool StagedToDevice( float* dDst, const float* src, int64_t count );
     };
 
     struct OutputRange
@@ -63,8 +78,19 @@ namespace fn2
     bool DecodeRange( const int32_t* record, int stride, float& min, float& max );
     // ints of a single call's range record
     constexpr int kCallRangeInts = kRangeSlots * kRangeStrideSpread;
-    bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err );
+    // Launch arguments of a spec (everything but the program and the output).
+    // A tileable spec uses `tileTable` (device floats laid out as FillTileTable
+    // writes them) when it is given, else the process's cached table of its
+    // size and step, which the first call creates.
+    bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err, const float* tileTable = nullptr );
+    // GenTileable2D's torus coordinates for xSize columns and ySize rows, on
+    // the host: [cos x | sin x] (2 * xSize floats), then [cos y | sin y]
+    void FillTileTable( int xSize, int ySize, float xStep, float yStep, float* table );
 
+    // Checks a compiled program before it reaches a kernel: instructions,
+    // register operands (regs = the register count it needs) and constant
+    // windows; false with `err` set when it is malformed
+    bool ValidateProgram( const Program& p, int& regs, std::string& err );
     const DeviceProgram* GetDeviceProgram( const Program& p );
     bool CompileForDevice( const FastNoise::Generator* root, int dims, const DeviceProgram*& dp, Program* hostCopy );
 

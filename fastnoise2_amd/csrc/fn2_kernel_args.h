@@ -27,6 +27,30 @@ namespace fn2
     constexpr int kQueueCounterStride = 32;
     constexpr int kQueueCounterInts = kQueueCounters * kQueueCounterStride;
 
+    // The work-queue partition of a launch of `waves` waves over `items`
+    // items: wave g runs item g first; the items beyond the first round are
+    // split into min(waves, kQueueCounters) contiguous parts, one per
+    // counter, and wave g pulls the items of part g % parts, [first, end),
+    // from that part's counter. Every part has a wave for any grid size.
+    struct QueuePart
+    {
+        uint32_t part, first, end;
+    };
+#if defined( __HIPCC__ ) || defined( __HIPCC_RTC__ )
+    __host__ __device__
+#endif
+    inline QueuePart queuePart( uint32_t g, uint32_t waves, uint32_t items )
+    {
+        const uint32_t parts = waves < (uint32_t)kQueueCounters ? waves : (uint32_t)kQueueCounters;
+        const uint32_t extra = items > waves ? items - waves : 0;
+        const uint32_t chunk = ( extra + parts - 1 ) / parts;
+        QueuePart q;
+        q.part = g % parts;
+        q.first = waves + q.part * chunk;
+        q.end = q.first + chunk < items ? q.first + chunk : items;
+        return q;
+    }
+
     struct GenArgs
     {
         const Ins* ins;          // device program

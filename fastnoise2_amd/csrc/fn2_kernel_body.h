@@ -258,9 +258,10 @@ namespace fn2k
     // item w covers the lanes (w % W) * 64 .. + 63 of tile w / W (W = waves
     // per block) exactly as that wave of block w / W runs it in
     // generateBatch. Wave g of the grid (G waves) runs item g first. Items
-    // beyond G are split into kQueueCounters contiguous parts; wave g then
-    // pulls the items of part g % kQueueCounters from that part's counter
-    // (zero at launch, kQueueCounterStride apart) until the part is done. A
+    // beyond G are split into min(G, kQueueCounters) contiguous parts
+    // (fn2::queuePart); wave g then pulls the items of its part from that
+    // part's counter (zero at launch, kQueueCounterStride apart) until the
+    // part is done. A
     // batch that fits the grid takes no atomic at all, and a large one
     // spreads its pulls over as many addresses (one counter for every
     // wave queued up at about 5 ns a pull, nearly doubling a large batch).
@@ -283,12 +284,9 @@ namespace fn2k
         const uint32_t waves = gridDim.x * kWaves;
         const int slot = (int)( g % fn2::kRangeSlots );
         // this wave's part of the items beyond the first round: [first, end)
-        const uint32_t part = g % fn2::kQueueCounters;
-        const uint32_t extra = items > waves ? items - waves : 0;
-        const uint32_t chunk = ( extra + fn2::kQueueCounters - 1 ) / fn2::kQueueCounters;
-        const uint32_t first = waves + part * chunk;
-        const uint32_t end = first + chunk < items ? first + chunk : items;
-        uint32_t* myCounter = counter + part * fn2::kQueueCounterStride;
+        const fn2::QueuePart q = fn2::queuePart( g, waves, items );
+        const uint32_t first = q.first, end = q.end;
+        uint32_t* myCounter = counter + q.part * fn2::kQueueCounterStride;
         for( uint32_t w = g; w < items; )
         {
             const uint32_t t = w / kWaves;

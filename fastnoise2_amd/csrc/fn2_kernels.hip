@@ -296,9 +296,10 @@ int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStr
 
 int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t* counts, hipStream_t stream )
 {
-    // interior voxels (gen - 2)^3 stay below 2^31 and every neighbour index
-    // inside the gen^3 block for 3 <= gen <= 1290
-    if( !density || !counts || gen < 3 || gen > 1290 ) return (int)hipErrorInvalidValue;
+    // the face count of a block reaches 6 (gen - 2)^3, counted in uint32:
+    // it stays below 2^32 for gen <= 896 (6 * 894^3 = 4,287,101,904), which
+    // also keeps every voxel and neighbour index inside the gen^3 block
+    if( !density || !counts || gen < 3 || gen > 896 ) return (int)hipErrorInvalidValue;
     const int64_t s = gen - 2, total = s * s * s;
     // at most a block per CU: every block adds into the same two counters,
     // and those atomics queue one behind the other (a 64^3 chunk would

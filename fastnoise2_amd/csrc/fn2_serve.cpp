@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "fn2_backend.h"
+#include "fn2_copy_pool.h"
 #include "fn2_internal.h"
 #include "fn2_jit.h"
 
@@ -80,7 +81,12 @@ namespace
         std::string compileError[5];
         int ppt[5] = {};   // tuned points per lane of the batched kernel, per dimensionality
         bool tunedSpecialised[5] = {};   // ppt timed on the tree's specialised batch kernel
+        bool tuning[5] = {};             // a submitter is timing this dimensionality now
     };
+
+    // points per lane of a request whose tree is being tuned by another
+    // submitter (any value gives the same bits)
+    constexpr int kUntunedPpt = 2;
 
     struct Request
     {
@@ -132,96 +138,6 @@ namespace
     };
 
     size_t AlignUp( size_t v, size_t a ) { return ( v + a - 1 ) / a * a; }
-
-    // Host-side copies of a retired batch (pinned staging -> callers' buffers)
-    // spread over a few threads: one thread's memcpy bandwidth would otherwise
-    // bound the engine at roughly 10 GB/s of output.
-    class CopyPool
-    {
-    public:
-        struct Seg { float* dst; const float* src; size_t n; };
-
-        explicit CopyPool( int threads )
-        {
-            for( int t = 0; t < threads; t++ ) workers.emplace_back( [this, t] { Work( t ); } );
-        }
-        ~CopyPool()
-        {
-            {
-                std::lock_guard<std::mutex> lk( m );
-                quit = true;
-                gen++;
-            }
-            cv.notify_all();
-            for( auto& w : workers ) w.join();
-        }
-        // copies every segment; returns when all are done
-        void Run( const std::vector<Seg>& segs )
-        {
-            size_t total = 0;
-            for( const Seg& g : segs ) total += g.n;
-            if( total < ( 1u << 18 ) || workers.empty() )
-            {
-                for( const Seg& g : segs ) memcpy( g.dst, g.src, g.n * 4 );
-                return;
-            }
-            // split into equal float ranges, one per worker plus the caller
-            parts.assign( workers.size() + 1, {} );
-            size_t per = ( total + parts.size() - 1 ) / parts.size();
-            size_t p = 0, fill = 0;
-            for( const Seg& g : segs )
-            {
-                size_t off = 0;
-                while( off < g.n )
-                {
-                    // the last part takes whatever remains
-                    size_t take = p + 1 < parts.size() ? std::min( g.n - off, per - fill ) : g.n - off;
-                    parts[p].push_back( { g.dst + off, g.src + off, take } );
-                    off += take;
-                    fill += take;
-                    if( fill == per && p + 1 < parts.size() ) { p++; fill = 0; }
-                }
-            }
-            {
-                std::lock_guard<std::mutex> lk( m );
-                pending = (int)workers.size();
-                gen++;
-            }
-            cv.notify_all();
-            for( const Seg& g : parts.back() ) memcpy( g.dst, g.src, g.n * 4 );
-            std::unique_lock<std::mutex> lk( m );
-            cvDone.wait( lk, [&] { return pending == 0; } );
-        }
-
-    private:
-        void Work( int t )
-        {
-            uint64_t seen = 0;
-            for( ;; )
-            {
-                {
-                    std::unique_lock<std::mutex> lk( m );
-                    cv.wait( lk, [&] { return gen != seen; } );
-                    seen = gen;
-                    if( quit ) return;
-                }
-                for( const Seg& g : parts[t] ) memcpy( g.dst, g.src, g.n * 4 );
-                {
-                    std::lock_guard<std::mutex> lk( m );
-                    pending--;
-                }
-                cvDone.notify_all();
-            }
-        }
-
-        std::vector<std::thread> workers;
-        std::vector<std::vector<Seg>> parts;
-        std::mutex m;
-        std::condition_variable cv, cvDone;
-        uint64_t gen = 0;
-        int pending = 0;
-        bool quit = false;
-    };
 
     class Engine
     {
@@ -305,7 +221,7 @@ namespace
                 }
                 deviceBytes += (int64_t)( (size_t)kTunePoints * 4 + sizeof( GenArgs ) + kTuneRangeInts * 4 + tb * 8 );
             }
-            copies.reset( new CopyPool( 3 ) );
+            copies.reset( new fn2::CopyPool( 3 ) );
             dispatcher = std::thread( [this] { DispatchLoop(); } );
             completer = std::thread( [this] { CompleteLoop(); } );
             return true;
@@ -423,29 +339,51 @@ namespace
                 prog = t.prog[d];
                 r->ppt = t.ppt[d];
             }
+            // One submitter per tree and dimensionality tunes (the flag is
+            // taken under the lock); others meanwhile run at the current or
+            // a default points per lane rather than time the tree again.
+            const int d = r->args.dims;
             if( r->ppt == 0 )
             {
-                int best = mode == FN2_SERVE_PLAIN ? 1 : Tune( prog, r->args.dims, nullptr );
-                if( best <= 0 ) return -1;
-                std::lock_guard<std::mutex> lk( mu );
-                trees[treeId]->ppt[r->args.dims] = best;
-                r->ppt = best;
+                bool mine = false;
+                if( mode == FN2_SERVE_PLAIN ) r->ppt = 1;
+                else
+                {
+                    std::lock_guard<std::mutex> lk( mu );
+                    mine = !trees[treeId]->tuning[d];
+                    trees[treeId]->tuning[d] = true;
+                }
+                if( mine )
+                {
+                    int best = Tune( prog, d, nullptr );
+                    std::lock_guard<std::mutex> lk( mu );
+                    trees[treeId]->tuning[d] = false;
+                    if( best <= 0 ) return -1;
+                    trees[treeId]->ppt[d] = best;
+                    r->ppt = best;
+                }
+                else if( r->ppt == 0 )
+                {
+                    r->ppt = kUntunedPpt;
+                }
             }
             if( mode != FN2_SERVE_PLAIN )
             {
                 // the first timing ran the interpreter; once the tree's
                 // specialised kernel is loaded, time that once more
-                const int d = r->args.dims;
                 bool retune = false;
                 {
                     std::lock_guard<std::mutex> lk( mu );
-                    retune = !trees[treeId]->tunedSpecialised[d];
+                    retune = !trees[treeId]->tunedSpecialised[d] && !trees[treeId]->tuning[d];
+                    if( retune && !fn2::GetJitKernel( prog ) ) retune = false;
+                    if( retune ) trees[treeId]->tuning[d] = true;
                 }
                 const fn2::JitKernel* k = retune ? fn2::GetJitKernel( prog ) : nullptr;
                 if( k )
                 {
                     int best = Tune( prog, d, k );
                     std::lock_guard<std::mutex> lk( mu );
+                    trees[treeId]->tuning[d] = false;
                     trees[treeId]->tunedSpecialised[d] = true;
                     if( best > 0 ) trees[treeId]->ppt[d] = best;
                     r->ppt = trees[treeId]->ppt[d];
@@ -531,6 +469,8 @@ namespace
         // and keeps the fastest. Runs in the thread that submits the tree's
         // first request of a dimensionality (on the interpreter) and again once
         // the tree's specialised kernel `jit` is loaded (on its batch kernel).
+        // The persistent mode's work-queue kernel runs the same per-lane loop
+        // over the same tiles, so the batch kernel's timing stands in for it.
         int Tune( const fn2::DeviceProgram* prog, int dims, const fn2::JitKernel* jit )
         {
             std::lock_guard<std::mutex> lk( tuneMu );
@@ -633,7 +573,7 @@ namespace
         int cus = 256;          // compute units of the device
         int nextSlot = 0;
         std::thread dispatcher, completer;
-        std::unique_ptr<CopyPool> copies;
+        std::unique_ptr<fn2::CopyPool> copies;
 
         // Fills slot s with queued requests (FIFO) up to its capacity; the plain
         // mode takes one request per launch. Called with mu held.
@@ -896,7 +836,7 @@ namespace
                 bool ok = code >= 0 && fn2::HipOk( hipEventSynchronize( s.done ), "serve batch" );
                 if( ok )
                 {
-                    std::vector<CopyPool::Seg> segs;
+                    std::vector<fn2::CopyPool::Seg> segs;
                     segs.reserve( s.reqs.size() );
                     for( Request* r : s.reqs )
                         if( r->dest == Request::kStaged ) segs.push_back( { r->hostOut, s.hOut + r->outOffset, (size_t)r->points } );

@@ -271,7 +271,8 @@ FASTNOISE_API int fn2WorldChunksInRange( const float* camera, int chunkSize, flo
 FASTNOISE_API void* fn2WorldCreate( void* engine, int treeId, int chunkSize, float isoSurface, float loadRange, int seed, int maxInFlight )
 {
     fn2::ClearError();
-    if( !engine || chunkSize < 1 || chunkSize > 1000 || maxInFlight < 1 || maxInFlight > 4096 ||
+    // chunks up to 894: the face count of a (size + 2)^3 block stays in uint32
+    if( !engine || chunkSize < 1 || chunkSize > 894 || maxInFlight < 1 || maxInFlight > 4096 ||
         !( loadRange >= 0.0f ) || loadRange > 65536.0f )
     {
         fn2::SetError( "bad world arguments" );

@@ -214,6 +214,13 @@ def bench_workloads():
         "serving_tree_3d_grid_256x256x256": dict(
             tree=serving_tree, kind="grid", dims=3, offset=[0, 0, 0], count=[256, 256, 256], seed=1337,
             ops=(1950.0, 4255.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
+        # cheap single-node grids: bounded by HBM writes, not by the ALU
+        "constant_2d_grid_16384x16384": dict(
+            tree=lambda: S("Constant"), kind="grid", dims=2, offset=[0, 0], count=[16384, 16384], seed=1337,
+            ops=(4.0, 2.0), ops_sample=dict(dims=2, offset=[0, 0], count=[64, 64])),
+        "checkerboard_2d_grid_16384x16384": dict(
+            tree=lambda: S("Checkerboard"), kind="grid", dims=2, offset=[0, 0], count=[16384, 16384], seed=1337,
+            ops=(10.0, 12.0), ops_sample=dict(dims=2, offset=[0, 0], count=[64, 64])),
         "mixed_tree_deepest_3d_grid_128x128x128": dict(
             tree=lambda: decode(mixed_trees()[-1]), kind="grid", dims=3, offset=[0, 0, 0], count=[128, 128, 128],
             seed=1337, ops=(24583.0, 41557.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),

@@ -199,7 +199,8 @@ typedef struct fn2ChunkResult
  * chunk, at most maxChunks written); returns the full count or -1. */
 FASTNOISE_API int fn2WorldChunksInRange( const float* camera, int chunkSize, float loadRange, int32_t* chunkOrigins, int maxChunks );
 /* A world over tree treeId of the engine, with at most maxInFlight chunks
- * requested at once (each holds a device density block). */
+ * requested at once (each holds a device density block); chunkSize 1..894
+ * (the face count of a block stays within 32 bits). */
 FASTNOISE_API void* fn2WorldCreate( void* engine, int treeId, int chunkSize, float isoSurface, float loadRange, int seed, int maxInFlight );
 FASTNOISE_API void fn2WorldDestroy( void* world );
 /* Unloads chunks beyond 1.1 x the load range and requests the missing ones in

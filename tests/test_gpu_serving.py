@@ -203,3 +203,47 @@ def test_persistent_queue_with_more_tiles_than_resident_blocks(gpu):
         assert mm == ref_mm
     assert eng.stats()["batches"] < len(reqs)
     eng.close()
+
+
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN, SERVE_PERSISTENT])
+def test_mixed_trees_alone_in_mix_and_oracle(gpu, mode):
+    # the 8 seeded random trees of the mixed serving workload (depth 3..12):
+    # requests of 2^10..2^16 points in 2D/3D/4D from four threads, each equal
+    # alone and in the mix; one small request per tree against the oracle
+    encs = W.mixed_trees()
+    nodes = [fn.Node.from_encoded(e) for e in encs]
+    rng = np.random.default_rng(90 + mode)
+    reqs = []
+    for i in range(48):
+        t = i % len(encs) if i < len(encs) else int(rng.integers(len(encs)))
+        d = int(rng.integers(2, 5))
+        pts = 2 ** rng.uniform(10, 12) if i < len(encs) else 2 ** rng.uniform(10, 16)
+        side = max(2, int(round(pts ** (1.0 / d))))
+        origin = [float(v) for v in rng.integers(-100000, 100000, d)]
+        reqs.append((t, fn.grid_spec(d, origin, [side] * d, seed=int(rng.integers(-2**31, 2**31)))))
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 20, mode=mode)
+    tids = [eng.add_tree(e) for e in encs]
+    ids = [None] * len(reqs)
+
+    def submitter(k):
+        for i in range(k, len(reqs), 4):
+            t, spec = reqs[i]
+            ids[i] = eng.submit(tids[t], spec)
+
+    threads = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    results = [eng.wait(i) for i in ids]
+    trees = [oracle.Tree(e) for e in encs]
+    for i, ((t, spec), (out, mm, _)) in enumerate(zip(reqs, results)):
+        ref, ref_mm = alone(nodes, t, spec)
+        assert_bitexact(out, ref, "mixed tree %d alone vs in the mix" % t)
+        assert mm == ref_mm
+        if i < len(encs):
+            d = spec.dims
+            oref, oref_mm = trees[t].grid(list(spec.offset)[:d], list(spec.count)[:d], list(spec.step)[:d], seed=spec.seed)
+            assert_bitexact(out, oref, "mixed tree %d vs oracle" % t)
+            assert mm == oref_mm
+    eng.close()
