@@ -91,6 +91,8 @@ C_ABI = {
 
 GPU_EXT = {
     "fn2GenerateDevice": (i, [vp, ctypes.POINTER(GenSpec), vp, c_float_p, vp]),
+    "fn2GenerateDeviceNoAlloc": (i, [vp, ctypes.POINTER(GenSpec), vp, c_float_p, vp]),
+    "fn2NoAllocScratchBytes": (ctypes.c_size_t, []),
     "fn2BenchGenerate": (i, [vp, ctypes.POINTER(GenSpec), vp, i, i, c_double_p, c_double_p]),
     "fn2Malloc": (vp, [ctypes.c_size_t]),
     "fn2Free": (None, [vp]),

@@ -45,6 +45,29 @@ typedef struct fn2GenSpec
  * fn2GetLastError() describes it. */
 FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, float* dOut, float* outputMinMax, void* stream );
 
+/* fn2GenerateDevice for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, hosts that forbid hipMalloc
+ * after start-up. No counterpart in the reference.
+ *
+ * The call allocates no device memory: the compiled program, the range
+ * record and GenTileable2D's torus table go into one device array of
+ * fn2NoAllocScratchBytes() bytes that each GPU holds until the process ends.
+ * It runs the interpreter kernel (the same bits as the specialised kernel).
+ * Limits: at most FN2_NOALLOC_MAX_CONSTANTS constants in the compiled tree
+ * and xSize + ySize <= FN2_NOALLOC_MAX_TILE_AXES for tileable specs; larger
+ * requests fail with an error and launch nothing. Positions must be device
+ * arrays. stream: a hipStream_t, or NULL for the calling thread's default
+ * stream (the call then waits for its work).
+ *
+ * Every call on a GPU rewrites that GPU's array: calls that overlap on one
+ * GPU race and must be serialised (one stream, or synchronised between).
+ * Returns 0 or a nonzero error code; fn2GetLastError() describes it. */
+#define FN2_NOALLOC_MAX_CONSTANTS 16384
+#define FN2_NOALLOC_MAX_TILE_AXES 32768
+FASTNOISE_API int fn2GenerateDeviceNoAlloc( const void* node, const fn2GenSpec* spec, float* dOut, float* outputMinMax, void* stream );
+/* Bytes of the per-GPU array fn2GenerateDeviceNoAlloc works in. */
+FASTNOISE_API size_t fn2NoAllocScratchBytes( void );
+
 /* Times `iters` back-to-back generator launches (each writes all points of
  * `spec` to dOut and folds them into the call's min/max) after `warmup`
  * launches, with HIP events on the launch stream. msPerLaunch: average event
