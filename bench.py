@@ -250,6 +250,10 @@ def roofline_for(name, wl, n, kernel_ms, counters, kernel_name, interp_ms=None):
         "hbm_algorithmic_GBs": round(hbm_gbs, 1),
         "hbm_peak_GBs": HBM_PEAK_GBS,
     })
+    if wl.get("pruned"):
+        r["note"] = ("ops per point count the reference's fixed 27-cell walk; the kernel's pruned walk leaves out "
+                     "cells that cannot change the result, so frac can exceed 1: hw.issue_frac is the share of the "
+                     "issue peak the kernel used")
     if interp_ms is not None:
         r["interpreter_kernel_ms"] = round(interp_ms, 4)
     if counters.get("FETCH_SIZE"):
@@ -262,6 +266,11 @@ def roofline_for(name, wl, n, kernel_ms, counters, kernel_name, interp_ms=None):
         # SQ_INSTS_VALU counts wave64 instructions: 64 lane operations each
         issued = c["SQ_INSTS_VALU"] * 64 / kernel_s / 1e12
         cycles = c.get("GRBM_GUI_ACTIVE", 0) / XCDS
+        if kernel_ms < 0.1:
+            # a profiled dispatch's active cycles include its launch and drain
+            # (several microseconds): for short kernels they say nothing of
+            # the kernel's own busy share or clock
+            cycles = 0
         r["hw"] = {
             "valu_instr_per_point": round(c["SQ_INSTS_VALU"] * 64 / n, 1),
             "issued_T_lane_ops_s": round(issued, 3),
@@ -277,6 +286,105 @@ def roofline_for(name, wl, n, kernel_ms, counters, kernel_name, interp_ms=None):
 def progress(msg):
     """One progress line on stderr per finished phase (the JSON line stays alone on stdout)."""
     print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def _clock_marks():
+    return {"monotonic": time.clock_gettime_ns(time.CLOCK_MONOTONIC),
+            "boottime": time.clock_gettime_ns(time.CLOCK_BOOTTIME)}
+
+
+def window(name, t0, t1, requests=None):
+    """Names a phase's wall-clock window on stderr, in both host clocks a
+    kernel trace may be stamped in, so that --account can sum the kernels
+    that ran inside it; `requests`: how many requests of the mixed trace
+    (from its start) the phase served."""
+    js = {k: [t0[k], t1[k]] for k in t0}
+    if requests is not None:
+        js["requests"] = requests
+    print("bench-window %s %s" % (name, json.dumps(js)), file=sys.stderr, flush=True)
+
+
+def account(trace_csv, log):
+    """Device-time accounting of the phases a bench run named (bench-window
+    lines in `log`) from a rocprofv3 --kernel-trace CSV of the same run:
+    kernel count, the sum of kernel durations, the time at least one kernel
+    was running (overlapping kernels on several streams counted once), both
+    against the window's wall time, and the idle gaps between busy spans.
+    For a window over the mixed trace's requests, also the algorithmic vector
+    ALU work of those requests (committed op counts per tree and
+    dimensionality) against the VALU issue peak over the wall, the busy time
+    and the kernel time; in plain mode (one launch per request, in submission
+    order) per tree as well."""
+    import csv
+    import fastnoise2_amd.workloads as W
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    wins = []
+    for line in open(log):
+        if line.startswith("bench-window "):
+            _, name, js = line.split(" ", 2)
+            wins.append((name, json.loads(js)))
+    out = []
+    mixed = None
+    for name, clocks in wins:
+        nreq = clocks.pop("requests", None)
+        # the clock whose window holds the most kernels is the trace's clock
+        best = max(clocks, key=lambda c: sum(1 for s, e, _ in rows if clocks[c][0] <= s and e <= clocks[c][1]))
+        t0, t1 = clocks[best]
+        ks = [(s, e, k) for s, e, k in rows if t0 <= s and e <= t1]
+        busy, gaps, cur_s, cur_e = 0, [], None, None
+        for s, e, _ in ks:
+            if cur_e is None or s > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                    gaps.append(s - cur_e)
+                cur_s, cur_e = s, e
+            else:
+                cur_e = max(cur_e, e)
+        if cur_e is not None:
+            busy += cur_e - cur_s
+        wall = t1 - t0
+        total = sum(e - s for s, e, _ in ks)
+        by = {}
+        for s, e, k in ks:
+            key = k.split("(")[0]
+            c = by.setdefault(key, [0, 0])
+            c[0] += 1
+            c[1] += e - s
+        g = np.array(gaps, dtype=np.float64) if gaps else np.zeros(1)
+        alg = {}
+        if nreq:
+            if mixed is None:
+                mixed = W.mixed_trace(2000)
+            trees, specs = mixed[0][:nreq], mixed[1][:nreq]
+            pts = np.array([float(np.prod(list(sp.count)[:sp.dims])) for sp in specs])
+            valu = np.array([W.MIXED_TREE_OPS[t][sp.dims][1] for t, sp in zip(trees, specs)]) * pts
+            peak = VALU_PEAK_TOPS * 1e12
+            gen = [(s, e) for s, e, k in ks if _is_generator(k) or k.startswith("fn2_jit_batch")]
+            alg = {"requests": nreq, "Gpoints": round(float(pts.sum()) / 1e9, 4),
+                   "valu_frac_over_wall": round(float(valu.sum()) / (wall * 1e-9) / peak, 3),
+                   "valu_frac_over_busy": round(float(valu.sum()) / (busy * 1e-9) / peak, 3) if busy else None,
+                   "valu_frac_over_kernel_sum": round(float(valu.sum()) / (total * 1e-9) / peak, 3) if total else None}
+            if name.endswith("-plain") and len(gen) == nreq:
+                dur = np.array([(e - s) * 1e-9 for s, e in gen])
+                tr = np.array(trees)
+                alg["per_tree"] = [{"tree": t, "requests": int((tr == t).sum()),
+                                    "kernel_ms": round(float(dur[tr == t].sum()) * 1e3, 3),
+                                    "Gpoints_s": round(float(pts[tr == t].sum()) / float(dur[tr == t].sum()) / 1e9, 3),
+                                    "valu_frac": round(float(valu[tr == t].sum()) / float(dur[tr == t].sum()) / peak, 3)}
+                                   for t in range(len(W.MIXED_TREE_OPS)) if (tr == t).any()]
+        out.append({"window": name, "clock": best, "wall_ms": round(wall / 1e6, 3), "kernels": len(ks),
+                    "kernel_sum_ms": round(total / 1e6, 3), "busy_ms": round(busy / 1e6, 3),
+                    "kernel_sum_over_wall": round(total / wall, 3), "busy_over_wall": round(busy / wall, 3),
+                    "gaps": len(gaps), "gap_ms_total": round(float(g.sum()) / 1e6, 3),
+                    "gap_us_p50": round(float(np.percentile(g, 50)) / 1e3, 1),
+                    "gap_us_p99": round(float(np.percentile(g, 99)) / 1e3, 1),
+                    "by_kernel": {k: {"count": c[0], "ms": round(c[1] / 1e6, 3)} for k, c in
+                                  sorted(by.items(), key=lambda kv: -kv[1][1])[:8]}, **alg})
+    return out
 
 
 def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=20000):
@@ -296,8 +404,10 @@ def serving_bench(W, trace_requests=20000, rate=2000.0, saturate_requests=20000)
     for name, mode in (("plain", SERVE_PLAIN), ("batched", SERVE_BATCHED), ("graph", SERVE_GRAPH), ("persistent", SERVE_PERSISTENT)):
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tid = eng.add_tree(enc)
-        eng.replay([tid] * 64, specs[:64], None, 16)   # compile + warm up
-        fn.jit_wait()   # specialised kernels ready before timing
+        # compile + warm up, then tune the tile on the specialised kernel
+        for _ in range(2):
+            eng.replay([tid] * 64, specs[:64], None, 16)
+            fn.jit_wait()
         rec = {}
         if name != "graph":
             lat, wall = eng.replay([tid] * len(specs), specs, arrivals, 512)
@@ -344,19 +454,27 @@ def serving_mixed_bench(W, trace_requests=2000, saturate_requests=1000, modes=("
         eng = ServeEngine(16 << 30, 1 << 23, mode)
         tids = [eng.add_tree(e) for e in encs]
         wt = [t for t in tids for _ in warm]
-        eng.replay(wt, warm * len(tids), None, 8)   # compile + tune every tree and dimensionality
-        fn.jit_wait()
+        # compile and tune every tree and dimensionality: the interpreter's
+        # tile, then (specialised kernels loaded) the specialised tile
+        for _ in range(2):
+            eng.replay(wt, warm * len(tids), None, 8)
+            fn.jit_wait()
         rec = {}
         lat, wall = eng.replay([tids[t] for t in trees], specs, arrivals, 512)
         rec.update({"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                     "trace_wall_s": round(wall, 2)})
         n = min(saturate_requests, len(specs))
+        w0 = _clock_marks()
         lat, wall = eng.replay([tids[t] for t in trees[:n]], specs[:n], None, 64)
+        window("mixed-saturated-" + name, w0, _clock_marks(), requests=n)
         rec.update({"saturated_Gpoints_s": round(float(pts[:n].sum()) / wall / 1e9, 3),
                     "saturated_p50_ms": round(float(np.percentile(lat, 50)), 3),
                     "saturated_p99_ms": round(float(np.percentile(lat, 99)), 3)})
         if mode != SERVE_PLAIN:
-            rec["tuned_points_per_lane"] = [[eng.tree_tuning(t, d) for d in (2, 3, 4)] for t in tids]
+            # per tree, per dimensionality 2/3/4: threads per block, points per
+            # lane, octave unroll (0 = the compiler's choice; the unroll stage
+            # is off by default)
+            rec["tuned_tiles"] = [[eng.tree_tile(t, d) for d in (2, 3, 4)] for t in tids]
         eng.close()
         out[name] = rec
         progress("mixed serving %s done" % name)
@@ -511,7 +629,13 @@ def main():
                     help="only run device-to-device copies of MIB MiB (the reference kernel for HBM counter scaling)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--account", nargs=2, metavar=("KERNEL_TRACE_CSV", "BENCH_STDERR"), default=None,
+                    help="no GPU: device-time accounting of the phases a bench run named on stderr, "
+                         "from a rocprofv3 --kernel-trace CSV of that run")
     args = ap.parse_args()
+    if args.account:
+        print(json.dumps(account(*args.account), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a single MI355X; "
               "splitting the index space over several GPUs is designed but not run (got --gpus %d)" % args.gpus, file=sys.stderr)

@@ -31,6 +31,8 @@ def device_count():
 
 
 JIT_OFF, JIT_ASYNC, JIT_SYNC = 0, 1, 2
+# variant of a specialised kernel that position arrays of 3D CellularDistance trees run
+JIT_VARIANT_SHARED_CELL_WALK = 4
 
 
 def set_jit_mode(mode):
@@ -270,9 +272,17 @@ class Node:
             raise RuntimeError("fn2BenchGenerate failed: " + last_error())
         return ms.value, wall.value
 
-    def jit_state(self, dims):
-        """-1 never requested, 0 queued, 1 ready, 2 failed, 3 kept on the interpreter (too long)."""
-        return lib().fn2GetJitState(self.handle, dims)
+    def jit_state(self, dims, variant=0):
+        """-1 never requested, 0 queued, 1 ready, 2 failed, 3 kept on the interpreter (too long).
+        variant: JIT_VARIANT_SHARED_CELL_WALK for the kernel position arrays of
+        3D CellularDistance trees run, 1..3 for the octave-unroll variants."""
+        return lib().fn2GetJitVariantState(self.handle, dims, variant)
+
+    def set_jit_variant(self, dims, variant):
+        """Octave-unroll variant (0 the compiler's choice, 1 none, 2 by two, 3 by
+        four) that single calls run in `dims` dimensions; same results."""
+        if lib().fn2SetJitVariant(self.handle, dims, variant):
+            raise ValueError("fn2SetJitVariant failed: " + last_error())
 
     def jit_source(self, dims):
         """HIP source of the tree's specialised kernel."""

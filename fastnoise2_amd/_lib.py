@@ -108,6 +108,8 @@ GPU_EXT = {
     "fn2JitCompileCheck": (i, [vp, i, c_int_p]),
     "fn2JitSource": (i, [vp, i, ctypes.c_char_p, i]),
     "fn2GetJitState": (i, [vp, i]),
+    "fn2GetJitVariantState": (i, [vp, i, i]),
+    "fn2SetJitVariant": (i, [vp, i, i]),
     "fn2GetProgramInfo": (i, [vp, i, c_int_p, c_int_p, c_int_p, c_int_p]),
     "fn2ServeCreate": (vp, [ctypes.c_size_t, i, i]),
     "fn2ServeDestroy": (None, [vp]),
@@ -117,6 +119,9 @@ GPU_EXT = {
     "fn2ServeDrain": (i, [vp]),
     "fn2ServeStats": (i, [vp, c_int64_p, c_int64_p, c_int64_p]),
     "fn2ServeTreeTuning": (i, [vp, i, i]),
+    "fn2ServeTuneOctaveUnroll": (i, [vp, i]),
+    "fn2ServeQueueStats": (i, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "fn2ServeTreeTile": (i, [vp, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
     "fn2ServePoll": (i, [vp, ctypes.c_int64, c_float_p, c_double_p]),
     "fn2WorldChunksInRange": (i, [c_float_p, i, ctypes.c_float, ctypes.POINTER(ctypes.c_int32), i]),
     "fn2WorldCreate": (vp, [vp, i, i, ctypes.c_float, ctypes.c_float, i, i]),

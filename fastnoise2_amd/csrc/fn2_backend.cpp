@@ -129,7 +129,15 @@ namespace fn2
     hipError_t LaunchProgram( const GenArgs& a, const DeviceProgram* dp, hipStream_t stream, bool* specialised )
     {
         if( a.total <= 0 ) return hipSuccess;
-        if( const JitKernel* k = GetJitKernel( dp ) )
+        const int chosen = dp->variant.load( std::memory_order_relaxed );
+        const JitKernel* k = GetJitKernel( dp );
+        if( chosen )
+            if( const JitKernel* v = GetJitKernel( dp, chosen ) ) k = v;
+        // scattered positions: the variant with the shared far-cell walk,
+        // once it is loaded
+        if( a.kind == SRC_POSITIONS && JitSharedWalkApplies( *dp ) )
+            if( const JitKernel* v = GetJitKernel( dp, chosen | kJitSharedWalk ) ) k = v;
+        if( k )
         {
             if( specialised ) *specialised = true;
             GenArgs copy = a;

@@ -2,6 +2,7 @@
 // the serving engine.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <string>
 #include <vector>
 #include "fn2_compiler.h"
@@ -24,6 +25,9 @@ namespace fn2
         int regs = 1;    // registers used: every operand lies below this
         Ins* dIns = nullptr;
         float* dKonst = nullptr;
+        // the specialised-kernel variant single calls run (fn2_jit.h; same
+        // results): set by fn2SetJitVariant
+        mutable std::atomic<int> variant{ 0 };
     };
 
     // Host buffers of at least this many floats move through the pinned

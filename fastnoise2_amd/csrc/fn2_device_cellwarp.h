@@ -295,6 +295,232 @@ namespace fn2d
         for( int k = 0; k < 4; k++ ) dist[k] = ibits( (int32_t)key[k] );
     }
 
+#ifndef FN2_SHARED_CELL_WALK
+#define FN2_SHARED_CELL_WALK 0
+#endif
+#if defined( __HIP_DEVICE_COMPILE__ )
+    // Per-wave scratch of the shared far-cell walk (cellularDistancesShared),
+    // one region per wave of a block of at most 256 threads: each lane's
+    // mirrored cell columns (3 x D offsets, 3 x D hash coordinates), its kept
+    // slots, and a queue of (lane, cell) items.
+    constexpr int kShareQueue = 192;
+    constexpr int kShareLaneWords = 3 * 3 * 2 + 2;
+    struct CellShare
+    {
+        uint32_t lane[64 * kShareLaneWords];
+        uint32_t queue[kShareQueue];
+    };
+    __device__ __forceinline__ CellShare& cellShare()
+    {
+        __shared__ CellShare waves[4];
+        return waves[threadIdx.x >> 6];
+    }
+
+    // The distance of the cell in mirrored columns o[] of a point whose
+    // columns are rel / hx ([axis * 3 + column]) (the walk body of the
+    // reference, Cellular.inl:391-409)
+    template<int D, typename REL, typename HX>
+    __device__ __forceinline__ uint32_t shareCellKey( const CellParams& cp, const int* o, REL rel, HX hx )
+    {
+        int32_t hc[D];
+        float d[D], v[D];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) hc[a] = hx( a, o[a] );
+        const int32_t h = cellHash<D>( cp.seed, hc );
+        cellVector<D>( h, v );
+        const float invMag = cp.jitter * invSqrtCell<D>( cellMagSq<D>( v ) );
+#pragma unroll
+        for( int a = 0; a < D; a++ ) d[a] = fmadd( v[a], invMag, rel( a, o[a] ) );
+        return (uint32_t)fbits( calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP ) );
+    }
+
+    // CellularDistance, squared distances, no size jitter, 2D/3D, one or two
+    // kept slots: the cells of the point's own octant (columns 1 and 2 of
+    // every axis, the near side) first, for every lane; then only the far
+    // cells each lane still needs, under the same bound as the pruned walk
+    // (a far cell whose lattice point lies beyond the kept slot's distance
+    // plus the jitter cannot change the slots). Where the lanes of a wave lie
+    // close together (grid bricks), the union of their needs is small and the
+    // wave walks it like the pruned walk does. Where they are scattered
+    // (position arrays), the union covers most far cells while each lane
+    // needs few; the lanes then queue their (lane, cell) items in LDS, every
+    // lane of the wave takes queued items in turn, and merges each distance
+    // into its owner's slots with LDS atomic minimums: slot 0 takes the
+    // minimum and slot 1 the minimum of what slot 0 gave up, which keeps the
+    // two smallest of all distances offered in any order. The slots end as
+    // those of the full walk, bit for bit.
+    // Only the kernel variant that position-array requests use compiles it
+    // in (FN2_SHARED_CELL_WALK, 3D): on MI355X it took the warped cellular
+    // tree over 16.8 M scattered positions from 0.334 to 0.311 ms, but the
+    // same tree on a 256^3 grid from 0.247 to 0.265 ms, where bricks already
+    // keep the wave's union small and its 23 KB of LDS per block and the
+    // extra scalar work cost occupancy and issue slots.
+    template<int D, int SLOTS>
+    __device__ __forceinline__ void cellularDistancesShared( const CellParams& cp, const float* pos, float* dist )
+    {
+        static_assert( D >= 2 && D <= 3 && SLOTS >= 1 && SLOTS <= 2, "shared walk: 2D/3D, one or two slots" );
+        using Order = MirroredCellOrder<D>;
+        constexpr Order order;
+        const int32_t prime[4] = { kPX, kPY, kPZ, kPW };
+        int32_t hx[D][3];
+        float rel[D][3];
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            const int32_t c = iadd( roundToInt( pos[a] ), -1 );
+            hx[a][0] = imul( c, prime[a] );
+            hx[a][1] = iadd( hx[a][0], prime[a] );
+            hx[a][2] = iadd( hx[a][1], prime[a] );
+            float fc = (float)c;
+            rel[a][0] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][1] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][2] = fc - pos[a];
+            // column 2 is the near neighbour, column 0 the far one
+            const bool below = rel[a][1] > 0.0f;
+            const int32_t h0 = hx[a][0], h2 = hx[a][2];
+            const float r0 = rel[a][0], r2 = rel[a][2];
+            hx[a][0] = below ? h2 : h0;
+            hx[a][2] = below ? h0 : h2;
+            rel[a][0] = below ? r2 : r0;
+            rel[a][2] = below ? r0 : r2;
+        }
+        auto relR = [&]( int a, int col ) { return rel[a][col]; };
+        auto hxR = [&]( int a, int col ) { return hx[a][col]; };
+        uint32_t key[2] = { 0x7f800000u, 0x7f800000u };   // +inf
+        auto insert = [&]( uint32_t nb ) {
+            if constexpr( SLOTS > 1 ) key[1] = umax( umin( key[1], nb ), key[0] );
+            key[0] = umin( key[0], nb );
+        };
+        // the near octant, for every lane
+#pragma unroll
+        for( int k = 0; k < Order::kCells; k++ )
+        {
+            const int code = order.e[k] & 0xff;
+            bool nearCell = true;
+            int o[D];
+#pragma unroll
+            for( int a = 0; a < D; a++ )
+            {
+                o[a] = ( code >> ( 2 * a ) ) & 3;
+                nearCell = nearCell && o[a] != 0;
+            }
+            if( nearCell ) insert( shareCellKey<D>( cp, o, relR, hxR ) );
+        }
+        // the far cells this lane can still need, as a mask over cell codes
+        const float jitterBound = fabs_( cp.jitter ) * ( 1.0f + 0x1p-16f );
+        const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+        const float limit = r * r;
+        // `need` per lane, `any` their union over the wave's active lanes
+        // (wave-uniform), both as masks over cell codes
+        uint64_t need = 0, any = 0;
+#pragma unroll
+        for( int k = 0; k < Order::kCells; k++ )
+        {
+            const int code = order.e[k] & 0xff;
+            bool nearCell = true;
+            float pp = 0.0f;
+#pragma unroll
+            for( int a = 0; a < D; a++ )
+            {
+                const int col = ( code >> ( 2 * a ) ) & 3;
+                nearCell = nearCell && col != 0;
+                pp = a == 0 ? rel[a][col] * rel[a][col] : fmadd( rel[a][col], rel[a][col], pp );
+            }
+            if( nearCell ) continue;
+            const bool mine = !( pp > limit );
+            need |= mine ? 1ull << code : 0ull;
+            any |= __ballot( mine ) ? 1ull << code : 0ull;
+        }
+        if( __builtin_popcountll( any ) <= 5 )
+        {
+            // few far cells for the whole wave: walk them together
+#pragma unroll
+            for( int k = 0; k < Order::kCells; k++ )
+            {
+                const int code = order.e[k] & 0xff;
+                if( !( any & ( 1ull << code ) ) ) continue;   // wave-uniform
+                int o[D];
+#pragma unroll
+                for( int a = 0; a < D; a++ ) o[a] = ( code >> ( 2 * a ) ) & 3;
+                insert( shareCellKey<D>( cp, o, relR, hxR ) );
+            }
+        }
+        else
+        {
+            CellShare& sh = cellShare();
+            const uint32_t lane = __lane_id();
+            uint32_t* mine = sh.lane + lane * kShareLaneWords;
+#pragma unroll
+            for( int a = 0; a < D; a++ )
+#pragma unroll
+                for( int col = 0; col < 3; col++ )
+                {
+                    mine[a * 3 + col] = (uint32_t)fbits( rel[a][col] );
+                    mine[9 + a * 3 + col] = (uint32_t)hx[a][col];
+                }
+            mine[18] = key[0];
+            mine[19] = key[1];
+            const uint64_t active = __ballot( 1 );
+            const uint32_t nActive = (uint32_t)__builtin_popcountll( active );
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi( (uint32_t)( active >> 32 ), __builtin_amdgcn_mbcnt_lo( (uint32_t)active, 0u ) );
+            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+            __builtin_amdgcn_wave_barrier();
+            // every active lane takes queued items in turn and merges each
+            // distance into its owner's slots
+            auto drain = [&]( uint32_t count ) {
+                __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+                __builtin_amdgcn_wave_barrier();
+                for( uint32_t q = rank; q < count; q += nActive )
+                {
+                    const uint32_t item = sh.queue[q];
+                    const uint32_t* owner = sh.lane + ( item >> 8 ) * kShareLaneWords;
+                    int o[D];
+#pragma unroll
+                    for( int a = 0; a < D; a++ ) o[a] = ( item >> ( 2 * a ) ) & 3;
+                    auto orel = [&]( int a, int col ) { return ibits( (int32_t)owner[a * 3 + col] ); };
+                    auto ohx = [&]( int a, int col ) { return (int32_t)owner[9 + a * 3 + col]; };
+                    const uint32_t nb = shareCellKey<D>( cp, o, orel, ohx );
+                    uint32_t* slots = sh.lane + ( item >> 8 ) * kShareLaneWords + 18;
+                    const uint32_t gave = atomicMin( &slots[0], nb );
+                    if constexpr( SLOTS > 1 ) atomicMin( &slots[1], umax( gave, nb ) );
+                }
+                __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+                __builtin_amdgcn_wave_barrier();
+            };
+            uint32_t queued = 0;   // wave-uniform
+            for( ;; )
+            {
+                const bool has = need != 0;
+                const uint64_t b = __ballot( has );
+                if( !b ) break;
+                const uint32_t n = (uint32_t)__builtin_popcountll( b );
+                if( queued + n > (uint32_t)kShareQueue )
+                {
+                    drain( queued );
+                    queued = 0;
+                }
+                if( has )
+                {
+                    const uint32_t code = (uint32_t)__builtin_ctzll( need );
+                    need &= need - 1;
+                    const uint32_t at = queued + __builtin_amdgcn_mbcnt_hi( (uint32_t)( b >> 32 ), __builtin_amdgcn_mbcnt_lo( (uint32_t)b, 0u ) );
+                    sh.queue[at] = ( lane << 8 ) | code;
+                }
+                queued += n;
+            }
+            drain( queued );
+            key[0] = mine[18];
+            key[1] = mine[19];
+            __builtin_amdgcn_wave_barrier();
+        }
+        dist[0] = ibits( (int32_t)key[0] );
+        dist[1] = SLOTS > 1 ? ibits( (int32_t)key[1] ) : __builtin_inff();
+        dist[2] = dist[3] = __builtin_inff();
+    }
+#endif
+
     // CellularDistance (Cellular.inl:294-529): the four smallest distances.
     // Slot k of the insertion network depends only on slots below k, so a
     // caller that reads slots < SLOTS only (distance indices known when the
@@ -308,6 +534,13 @@ namespace fn2d
         {
             if( squaredDistance( cp.distFunc ) && !cp.sizeJitterOn )
             {
+#if defined( __HIP_DEVICE_COMPILE__ ) && FN2_SHARED_CELL_WALK
+                if constexpr( D == 3 && SLOTS <= 2 )
+                {
+                    cellularDistancesShared<D, SLOTS>( cp, pos, dist );
+                    return;
+                }
+#endif
                 cellularDistancesPruned<D, SLOTS>( cp, pos, dist );
                 return;
             }

@@ -19,6 +19,11 @@
 #define FN2_PRUNED_CELL_WALK false
 #endif
 
+// Unroll factor of a fused fractal's octave loop; 0 leaves it to the compiler
+#ifndef FN2_OCTAVE_UNROLL
+#define FN2_OCTAVE_UNROLL 0
+#endif
+
 namespace fn2i
 {
     using namespace fn2d;
@@ -98,6 +103,9 @@ namespace fn2i
 
     // FractalFBm / FractalRidged over a lattice leaf with constant gain and
     // weighted strength (Fractal.inl:9-67); the octave loop stays in registers.
+    // A specialised kernel knows the octave count; FN2_OCTAVE_UNROLL (0: the
+    // compiler's choice) sets how far the loop is unrolled, a per-tree tuning
+    // knob of the serving engine.
     template<int D, int LEAF = -1, class RF = const float*>
     FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, RF R, int p )
     {
@@ -118,6 +126,9 @@ namespace fn2i
         float noise = octave( seed );
         if( ridged ) noise = fabs_( noise );
         float sum = ridged ? ( noise * -2.0f + 1.0f ) * amp : noise * amp;
+#if FN2_OCTAVE_UNROLL > 0
+#pragma unroll FN2_OCTAVE_UNROLL
+#endif
         for( int i = 1; i < octaves; i++ )
         {
             seed = iadd( seed, 1 );

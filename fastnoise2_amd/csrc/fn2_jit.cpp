@@ -102,9 +102,11 @@ namespace fn2
         // Each instruction becomes an `execIns` instantiated for its own op,
         // dimensionality and (fused fractal) leaf op, so the compiler sees only
         // the code that instruction runs.
-        std::string Source( const DeviceProgram& p )
+        std::string Source( const DeviceProgram& p, int variant )
         {
             std::ostringstream s;
+            if( variant & kJitUnrollMask ) s << "#define FN2_OCTAVE_UNROLL " << kJitUnrollFactor[variant & kJitUnrollMask] << "\n";
+            if( variant & kJitSharedWalk ) s << "#define FN2_SHARED_CELL_WALK 1\n";
             s << "#include \"fn2_kernel_body.h\"\n"
                  "using namespace fn2d;\nusing namespace fn2i;\n"
                  "namespace {\nstruct Prog {\n"
@@ -149,9 +151,9 @@ namespace fn2
             return s.str();
         }
 
-        bool CompileCode( const DeviceProgram& p, std::vector<char>& code, std::string& err )
+        bool CompileCode( const DeviceProgram& p, int variant, std::vector<char>& code, std::string& err )
         {
-            std::string src = Source( p );
+            std::string src = Source( p, variant );
             std::vector<const char*> hdrs, names;
             for( const JitHeader& h : kJitHeaders )
             {
@@ -183,10 +185,10 @@ namespace fn2
             return true;
         }
 
-        bool Compile( const DeviceProgram& p, JitKernel& k, std::string& err )
+        bool Compile( const DeviceProgram& p, int variant, JitKernel& k, std::string& err )
         {
             std::vector<char> code;
-            if( !CompileCode( p, code, err ) ) return false;
+            if( !CompileCode( p, variant, code, err ) ) return false;
             hipModule_t mod = nullptr;
             hipError_t e = hipModuleLoadData( &mod, code.data() );
             if( e == hipSuccess ) e = hipModuleGetFunction( &k.single, mod, "fn2_jit_single" );
@@ -212,6 +214,7 @@ namespace fn2
             return true;
         }
 
+        // One per process, never destroyed (see Manager)
         class JitManager
         {
         public:
@@ -229,20 +232,20 @@ namespace fn2
                 if( th && atoi( th ) > 0 ) threads = std::min( 16, atoi( th ) );
             }
 
-            ~JitManager() { Shutdown(); }
 
-            // Stops the background compiler (the compilation in progress
-            // finishes first); registered with atexit so that the thread never
-            // outlives the compiler libraries it calls.
+            // Stops the background compiler at exit (registered with atexit):
+            // no queued compilation starts, and the ones in progress finish, so
+            // no worker is inside the compiler libraries while they are torn
+            // down. Idle workers are not woken: they stay blocked until the
+            // process ends, so none of them exits, and runs the HIP runtime's
+            // per-thread clean-up, while the runtime's own state is destroyed.
             void Shutdown()
             {
-                {
-                    std::lock_guard<std::mutex> lk( mu );
-                    stop = true;
-                }
-                cv.notify_all();
+                std::unique_lock<std::mutex> lk( mu );
+                stop = true;
+                cv.wait( lk, [&] { return busy == 0; } );
                 for( std::thread& w : workers )
-                    if( w.joinable() ) w.join();
+                    if( w.joinable() ) w.detach();
             }
 
             int mode = FN2_JIT_ASYNC;
@@ -251,14 +254,15 @@ namespace fn2
             // MI355X host, most of it in LLVM's value-range propagation)
             size_t maxInstructions = 512;
 
-            const JitKernel* Get( const DeviceProgram* p )
+            const JitKernel* Get( const DeviceProgram* p, int variant )
             {
                 if( mode == FN2_JIT_OFF || !p ) return nullptr;
+                const Key key{ p, variant };
                 std::unique_lock<std::mutex> lk( mu );
-                auto it = entries.find( p );
+                auto it = entries.find( key );
                 if( it == entries.end() )
                 {
-                    it = entries.emplace( p, Entry() ).first;
+                    it = entries.emplace( key, Entry() ).first;
                     if( p->ins.size() > maxInstructions )
                     {
                         it->second.state = kTooLarge;
@@ -268,16 +272,16 @@ namespace fn2
                     {
                         lk.unlock();
                         Entry e;
-                        e.state = Compile( *p, e.kernel, e.error ) ? kReady : kFailed;
+                        e.state = Compile( *p, variant, e.kernel, e.error ) ? kReady : kFailed;
                         lk.lock();
-                        it = entries.find( p );
+                        it = entries.find( key );
                         it->second = e;
                         cv.notify_all();
                     }
                     else
                     {
                         if( stop ) return nullptr;
-                        queue.push_back( p );
+                        queue.push_back( key );
                         // compilations are independent: a few run side by side
                         if( (int)workers.size() < threads && (int)workers.size() < (int)queue.size() + busy )
                             workers.emplace_back( [this] { Work(); } );
@@ -287,16 +291,16 @@ namespace fn2
                 }
                 else if( mode == FN2_JIT_SYNC && it->second.state == kQueued )
                 {
-                    cv.wait( lk, [&] { return entries[p].state != kQueued; } );
-                    it = entries.find( p );
+                    cv.wait( lk, [&] { return entries[key].state != kQueued; } );
+                    it = entries.find( key );
                 }
                 return it->second.state == kReady ? &it->second.kernel : nullptr;
             }
 
-            int State( const DeviceProgram* p, std::string* err )
+            int State( const DeviceProgram* p, int variant, std::string* err )
             {
                 std::lock_guard<std::mutex> lk( mu );
-                auto it = entries.find( p );
+                auto it = entries.find( Key{ p, variant } );
                 if( it == entries.end() ) return -1;
                 if( err ) *err = it->second.error;
                 return it->second.state;
@@ -313,30 +317,41 @@ namespace fn2
             {
                 for( ;; )
                 {
-                    const DeviceProgram* p;
+                    Key key;
                     {
                         std::unique_lock<std::mutex> lk( mu );
-                        cv.wait( lk, [&] { return stop || !queue.empty(); } );
-                        if( stop ) return;
-                        p = queue.front();
+                        // after Shutdown, an idle worker waits until the process ends
+                        cv.wait( lk, [&] { return !stop && !queue.empty(); } );
+                        key = queue.front();
                         queue.pop_front();
                         busy++;
                     }
                     Entry e;
-                    e.state = Compile( *p, e.kernel, e.error ) ? kReady : kFailed;
+                    e.state = Compile( *key.p, key.variant, e.kernel, e.error ) ? kReady : kFailed;
                     {
                         std::lock_guard<std::mutex> lk( mu );
-                        entries[p] = e;
+                        entries[key] = e;
                         busy--;
                     }
                     cv.notify_all();
                 }
             }
 
+            struct Key
+            {
+                const DeviceProgram* p = nullptr;
+                int variant = 0;
+                bool operator==( const Key& o ) const { return p == o.p && variant == o.variant; }
+            };
+            struct KeyHash
+            {
+                size_t operator()( const Key& k ) const { return std::hash<const void*>()( k.p ) * 31u + (size_t)k.variant; }
+            };
+
             std::mutex mu;
             std::condition_variable cv;
-            std::unordered_map<const DeviceProgram*, Entry> entries;
-            std::deque<const DeviceProgram*> queue;
+            std::unordered_map<Key, Entry, KeyHash> entries;
+            std::deque<Key> queue;
             std::vector<std::thread> workers;
             int threads = 4;
             int busy = 0;
@@ -352,8 +367,8 @@ namespace fn2
 
         JitManager& Manager()
         {
-            // lives for the process (its kernels stay loaded); the worker
-            // thread is stopped at exit
+            // lives for the process (its kernels stay loaded); the workers
+            // stop taking compilations at exit
             static JitManager* m = [] {
                 gManager = new JitManager;
                 std::atexit( ShutdownAtExit );
@@ -363,14 +378,34 @@ namespace fn2
         }
     }
 
-    const JitKernel* GetJitKernel( const DeviceProgram* p )
+    const JitKernel* GetJitKernel( const DeviceProgram* p, int variant )
     {
-        return Manager().Get( p );
+        return Manager().Get( p, variant );
     }
 
-    std::string JitSourceFor( const DeviceProgram& p )
+    int JitVariantState( const DeviceProgram* p, int variant )
     {
-        return Source( p );
+        return Manager().State( p, variant, nullptr );
+    }
+
+    bool JitHasFusedFractal( const DeviceProgram& p )
+    {
+        for( const Ins& in : p.ins )
+            if( in.op == OP_FRACTAL_LEAF ) return true;
+        return false;
+    }
+
+    bool JitSharedWalkApplies( const DeviceProgram& p )
+    {
+        for( const Ins& in : p.ins )
+            if( in.op == OP_CELL_DISTANCE && in.dim == 3 && ( in.p[0] < 2 || in.p[0] > 5 ) && in.r[3] == kNoReg && Aux( in ) <= 2 )
+                return true;
+        return false;
+    }
+
+    std::string JitSourceFor( const DeviceProgram& p, int variant )
+    {
+        return Source( p, variant );
     }
 }
 
@@ -407,7 +442,7 @@ FASTNOISE_API int fn2JitCompileCheck( const void* node, int dims, int* codeBytes
     dp.resultReg = prog.resultReg;
     dp.dims = prog.dims;
     std::vector<char> code;
-    if( !fn2::CompileCode( dp, code, err ) )
+    if( !fn2::CompileCode( dp, 0, code, err ) )
     {
         fn2::SetError( err );
         return 1;
@@ -432,7 +467,7 @@ FASTNOISE_API int fn2JitSource( const void* node, int dims, char* buf, int bufSi
     dp.konst = prog.konst;
     dp.resultReg = prog.resultReg;
     dp.dims = prog.dims;
-    std::string src = fn2::Source( dp );
+    std::string src = fn2::Source( dp, 0 );
     if( buf && bufSize > 0 )
     {
         size_t n = std::min( src.size(), (size_t)bufSize - 1 );
@@ -442,16 +477,35 @@ FASTNOISE_API int fn2JitSource( const void* node, int dims, char* buf, int bufSi
     return (int)src.size();
 }
 
-FASTNOISE_API int fn2GetJitState( const void* node, int dims )
+FASTNOISE_API int fn2GetJitVariantState( const void* node, int dims, int variant )
 {
     fn2::ClearError();
-    if( !node ) return -2;
+    if( !node || variant < 0 || variant > ( fn2::kJitUnrollMask | fn2::kJitSharedWalk ) ) return -2;
     const fn2::DeviceProgram* dp = nullptr;
     if( !fn2::EnsureDevice() || !fn2::CompileForDevice( static_cast<const FastNoise::Generator*>( node ), dims, dp, nullptr ) ) return -2;
     std::string err;
-    int st = fn2::Manager().State( dp, &err );
+    int st = fn2::Manager().State( dp, variant, &err );
     if( st == 2 ) fn2::SetError( err );
     return st;
+}
+
+FASTNOISE_API int fn2SetJitVariant( const void* node, int dims, int variant )
+{
+    fn2::ClearError();
+    if( !node || variant < 0 || variant > fn2::kJitUnrollMask )
+    {
+        fn2::SetError( "bad node or variant" );
+        return 1;
+    }
+    const fn2::DeviceProgram* dp = nullptr;
+    if( !fn2::EnsureDevice() || !fn2::CompileForDevice( static_cast<const FastNoise::Generator*>( node ), dims, dp, nullptr ) ) return 1;
+    dp->variant.store( variant, std::memory_order_relaxed );
+    return 0;
+}
+
+FASTNOISE_API int fn2GetJitState( const void* node, int dims )
+{
+    return fn2GetJitVariantState( node, dims, 0 );
 }
 
 }

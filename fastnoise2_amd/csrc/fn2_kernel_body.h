@@ -17,6 +17,8 @@ namespace fn2k
     using namespace fn2d;
     using namespace fn2i;
 
+    // BLOCK: the most threads a block of the kernel has (its launch bound);
+    // the block may have fewer, any multiple of 64.
     template<int BLOCK>
     __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* record, int stride )
     {
@@ -37,9 +39,11 @@ namespace fn2k
         __syncthreads();
         if( threadIdx.x == 0 )
         {
+            const int waves = (int)( blockDim.x >> 6 );
 #pragma unroll
             for( int w = 1; w < BLOCK / 64; w++ )
             {
+                if( w >= waves ) break;
                 mn = fminf( mn, sMin[w] );
                 mx = fmaxf( mx, sMax[w] );
             }
@@ -173,9 +177,11 @@ namespace fn2k
         if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride );
     }
 
-    // One tile of a batched request: points [first, first + BLOCK * ppt) of
-    // job a (clipped to its size), where ppt is the job's points per lane;
-    // the tile's range goes to the job's key pair.
+    // One tile of a batched request: points [first, first + threads * ppt)
+    // of job a (clipped to its size), where threads is the block's size (at
+    // most BLOCK; the interpreter's register file in LDS is laid out for
+    // BLOCK) and ppt the job's points per lane; the tile's range goes to the
+    // job's key pair.
     template<int BLOCK, class Prog>
     __device__ __forceinline__ void generateTile( const fn2::GenArgs& a, uint32_t first, Prog prog )
     {
@@ -190,13 +196,14 @@ namespace fn2k
         };
         const uint32_t total = (uint32_t)a.total;
         const uint32_t base = first + threadIdx.x;
+        const uint32_t threads = blockDim.x;
         const int ppt = a.ppt;
         if constexpr( Prog::kFresh )
         {
 #pragma unroll 1
             for( int p = 0; p < ppt; p++ )
             {
-                const uint32_t i = base + p * BLOCK;
+                const uint32_t i = base + p * threads;
                 if( i < total )
                 {
                     RegFile fresh;
@@ -211,7 +218,7 @@ namespace fn2k
 #pragma unroll 1
             for( int p = 0; p < ppt; p++ )
             {
-                const uint32_t i = base + p * BLOCK;
+                const uint32_t i = base + p * threads;
                 if( i < total )
                 {
                     startPosition( a, i, R );

@@ -68,8 +68,11 @@ namespace
     const int kThreads = fn2_batch_block_threads();
     // the tuner's range record: compact
     constexpr int kTuneRangeInts = fn2::kRangeSlots * fn2::kRangeStrideCompact;
-    // candidate points per lane of the tuner, and its calibration size
+    // the tuner's candidates: points per lane, threads per block (the
+    // specialised batch kernel; the interpreter's register file in LDS is laid
+    // out for kThreads), and its calibration size
     constexpr int kPptCandidates[] = { 1, 2, 4, 8 };
+    constexpr int kThreadCandidates[] = { 64, 128, 256 };
     constexpr int kTunePoints = 1 << 20;
     constexpr int kSlots = 3;
     constexpr int kMaxJobs = 4096;
@@ -79,9 +82,24 @@ namespace
         FastNoise::SmartNode<> node;
         const fn2::DeviceProgram* prog[5] = {};
         std::string compileError[5];
-        int ppt[5] = {};   // tuned points per lane of the batched kernel, per dimensionality
-        bool tunedSpecialised[5] = {};   // ppt timed on the tree's specialised batch kernel
+        // the tuned tile of the batched kernel, per dimensionality: points per
+        // lane, threads per block, and the variant of the specialised kernel
+        // (its octave-loop unroll, fn2_jit.h)
+        int ppt[5] = {};
+        int threads[5] = { 256, 256, 256, 256, 256 };
+        int variant[5] = {};
+        bool tunedSpecialised[5] = {};   // tile timed on the tree's specialised batch kernel
+        int unrollStage[5] = {};         // 0 not started, 1 variants requested, 2 decided
+        float tileMs[5] = {};            // the tuned tile's time on the default specialised kernel
         bool tuning[5] = {};             // a submitter is timing this dimensionality now
+    };
+
+    // A tile of the batched kernel: threads per block x points per lane
+    struct Tile
+    {
+        int threads = 256;
+        int ppt = 0;
+        float ms = 0.0f;   // the tuner's best time for its 2^20-point request
     };
 
     // points per lane of a request whose tree is being tuned by another
@@ -102,6 +120,8 @@ namespace
         size_t outOffset = 0;
         int job = 0;
         int ppt = 1;
+        int threads = 256;   // threads per block when a specialised kernel runs it
+        int variant = 0;     // variant of the specialised kernel
         const fn2::JitKernel* jit = nullptr;
         // where the values go: through the slot's pinned staging buffer (pageable
         // caller memory), straight from the arena into pinned caller memory, or
@@ -133,8 +153,11 @@ namespace
         // graph mode
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        std::vector<std::array<int64_t, 3>> graphShape;   // (kernel key, blocks, register file) per launch
+        std::vector<std::array<int64_t, 4>> graphShape;   // (kernel key, blocks, register file, threads) per launch
         int graphJobs = -1;
+        // persistent mode: work-queue launches of the batch in flight, whose
+        // counters come back with the range keys
+        int queueLaunches = 0;
     };
 
     size_t AlignUp( size_t v, size_t a ) { return ( v + a - 1 ) / a * a; }
@@ -163,7 +186,7 @@ namespace
         {
             if( capacity < kThreads ) capacity = kThreads;
             capacity = (int64_t)AlignUp( (size_t)capacity, kThreads );
-            maxBlocks = (int)( capacity / kThreads ) + kMaxJobs;
+            maxBlocks = (int)( capacity / kThreadCandidates[0] ) + kMaxJobs;   // the smallest tiles
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             size_t bbBytes = AlignUp( sizeof( uint32_t ) * maxBlocks, 256 );
@@ -185,7 +208,7 @@ namespace
                     !fn2::HipOk( hipMalloc( (void**)&s.dOut, (size_t)capacity * 4 ), "serve arena" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hOut, (size_t)capacity * 4, hipHostMallocDefault ), "serve staging" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dKeys, keyBytes ), "serve keys" ) ||
-                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, jobRangeInts * sizeof( int32_t ) * kMaxJobs, hipHostMallocDefault ), "serve keys" ) ||
+                    !fn2::HipOk( hipHostMalloc( (void**)&s.hKeys, keyBytes, hipHostMallocDefault ), "serve keys" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&s.dJobs, s.tableBytes ), "serve tables" ) ||
                     !fn2::HipOk( hipHostMalloc( (void**)&s.hTables, s.tableBytes, hipHostMallocDefault ), "serve tables" ) )
                 {
@@ -208,7 +231,7 @@ namespace
             }
             if( mode != FN2_SERVE_PLAIN )
             {
-                size_t tb = (size_t)kTunePoints / kThreads + 1;
+                size_t tb = (size_t)kTunePoints / kThreadCandidates[0] + 1;   // blocks of the smallest tile
                 if( !fn2::HipOk( hipStreamCreateWithFlags( &tuneStream, hipStreamNonBlocking ), "tune stream" ) ||
                     !fn2::HipOk( hipEventCreate( &tuneEv0 ), "tune event" ) || !fn2::HipOk( hipEventCreate( &tuneEv1 ), "tune event" ) ||
                     !fn2::HipOk( hipMalloc( (void**)&tuneOut, (size_t)kTunePoints * 4 ), "tune arena" ) ||
@@ -268,6 +291,17 @@ namespace
             std::lock_guard<std::mutex> lk( mu );
             if( treeId < 0 || treeId >= (int)trees.size() || dims < 2 || dims > 4 ) return -1;
             return trees[treeId]->ppt[dims];
+        }
+
+        int TreeTile( int treeId, int dims, int* threads, int* ppt, int* unroll )
+        {
+            std::lock_guard<std::mutex> lk( mu );
+            if( treeId < 0 || treeId >= (int)trees.size() || dims < 2 || dims > 4 ) return -1;
+            const Tree& t = *trees[treeId];
+            if( threads ) *threads = t.threads[dims];
+            if( ppt ) *ppt = t.ppt[dims];
+            if( unroll ) *unroll = fn2::kJitUnrollFactor[t.variant[dims] & fn2::kJitUnrollMask];
+            return t.ppt[dims] ? 0 : 1;
         }
 
         int AddTree( const char* encoded )
@@ -338,6 +372,8 @@ namespace
                 }
                 prog = t.prog[d];
                 r->ppt = t.ppt[d];
+                r->threads = t.threads[d];
+                r->variant = t.variant[d];
             }
             // One submitter per tree and dimensionality tunes (the flag is
             // taken under the lock); others meanwhile run at the current or
@@ -355,12 +391,12 @@ namespace
                 }
                 if( mine )
                 {
-                    int best = Tune( prog, d, nullptr );
+                    Tile best = Tune( prog, d, nullptr, false );
                     std::lock_guard<std::mutex> lk( mu );
                     trees[treeId]->tuning[d] = false;
-                    if( best <= 0 ) return -1;
-                    trees[treeId]->ppt[d] = best;
-                    r->ppt = best;
+                    if( best.ppt <= 0 ) return -1;
+                    trees[treeId]->ppt[d] = best.ppt;
+                    r->ppt = best.ppt;
                 }
                 else if( r->ppt == 0 )
                 {
@@ -381,13 +417,28 @@ namespace
                 const fn2::JitKernel* k = retune ? fn2::GetJitKernel( prog ) : nullptr;
                 if( k )
                 {
-                    int best = Tune( prog, d, k );
+                    // the tile over threads per block and points per lane,
+                    // except in persistent mode, whose work-queue kernel pulls
+                    // quarter tiles of full blocks
+                    Tile best = Tune( prog, d, k, mode != FN2_SERVE_PERSISTENT );
                     std::lock_guard<std::mutex> lk( mu );
-                    trees[treeId]->tuning[d] = false;
-                    trees[treeId]->tunedSpecialised[d] = true;
-                    if( best > 0 ) trees[treeId]->ppt[d] = best;
-                    r->ppt = trees[treeId]->ppt[d];
+                    Tree& t = *trees[treeId];
+                    t.tuning[d] = false;
+                    t.tunedSpecialised[d] = true;
+                    if( best.ppt > 0 )
+                    {
+                        t.ppt[d] = best.ppt;
+                        t.threads[d] = best.threads;
+                        t.tileMs[d] = best.ms;
+                    }
+                    // a fused fractal's octave loop (when enabled): compile the
+                    // unroll variants in the background, time them once loaded
+                    if( tuneUnroll && mode != FN2_SERVE_PERSISTENT && fn2::JitHasFusedFractal( *prog ) ) t.unrollStage[d] = 1;
+                    else t.unrollStage[d] = 2;
+                    r->ppt = t.ppt[d];
+                    r->threads = t.threads[d];
                 }
+                TuneUnroll( treeId, d, prog, r.get() );
             }
             r->prog = prog;
             r->args.ins = prog->dIns;
@@ -463,34 +514,137 @@ namespace
             if( p ) *p = deviceBytes;
         }
 
+        // The octave-unroll stage of the tuner: off by default. On the mixed
+        // trees it found 2-8 % on three of the five trees with fused fractals
+        // and nothing on the others, with the same saturated throughput,
+        // while it costs three more background compilations per tree and
+        // dimensionality.
+        void SetTuneUnroll( bool on )
+        {
+            std::lock_guard<std::mutex> lk( mu );
+            tuneUnroll = on;
+        }
+
+        void QueueStats( int64_t* launches, int64_t* pulledLaunches, int64_t* pulls )
+        {
+            std::lock_guard<std::mutex> lk( mu );
+            if( launches ) *launches = queueLaunches;
+            if( pulledLaunches ) *pulledLaunches = queuePulledLaunches;
+            if( pulls ) *pulls = queuePulls;
+        }
+
     private:
         // Per-tree tuning of the batched kernel: times a 2^20-point request of
-        // the program at each candidate points-per-lane on the tuning stream
-        // and keeps the fastest. Runs in the thread that submits the tree's
-        // first request of a dimensionality (on the interpreter) and again once
-        // the tree's specialised kernel `jit` is loaded (on its batch kernel).
-        // The persistent mode's work-queue kernel runs the same per-lane loop
-        // over the same tiles, so the batch kernel's timing stands in for it.
-        int Tune( const fn2::DeviceProgram* prog, int dims, const fn2::JitKernel* jit )
+        // the program at each candidate tile on the tuning stream and keeps
+        // the fastest (a candidate must beat the best so far by 3 %; the
+        // candidates run from small to large tiles). Runs in the thread that
+        // submits the tree's first request of a dimensionality (on the
+        // interpreter, points per lane only) and again once the tree's
+        // specialised kernel `jit` is loaded (on its batch kernel, with the
+        // threads per block as well when `threadsToo`). The persistent mode's
+        // work-queue kernel runs the same per-lane loop over the same tiles,
+        // so the batch kernel's timing stands in for it.
+        Tile Tune( const fn2::DeviceProgram* prog, int dims, const fn2::JitKernel* jit, bool threadsToo )
         {
             std::lock_guard<std::mutex> lk( tuneMu );
+            GenArgs a{};
+            int64_t n = 0;
+            Tile best;
+            if( !TuneArgs( prog, dims, a, n ) ) return best;
+            best.ms = 1e30f;
+            for( int threads : kThreadCandidates )
+            {
+                if( threads != kThreads && !( jit && threadsToo ) ) continue;
+                for( int ppt : kPptCandidates )
+                {
+                    float ms = TimeTile( prog, a, n, jit, threads, ppt );
+                    if( ms < 0 ) return Tile();
+                    if( ms < best.ms * 0.97f )
+                    {
+                        best.ms = ms;
+                        best.ppt = ppt;
+                        best.threads = threads;
+                    }
+                }
+            }
+            return best;
+        }
+
+        // The octave-unroll stage of a tree's tuning (after its tile is tuned on
+        // the default specialised kernel): requests the unroll variants, and
+        // once every one has finished compiling, times the loaded ones at the
+        // tuned tile and keeps the fastest if it beats the default by 3 %.
+        // Called on every submit of the tree and dimensionality until decided;
+        // never blocks on a compilation in async mode.
+        void TuneUnroll( int treeId, int d, const fn2::DeviceProgram* prog, Request* r )
+        {
+            int threads, ppt;
+            float baseMs;
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                Tree& t = *trees[treeId];
+                if( t.unrollStage[d] != 1 || t.tuning[d] ) return;
+                threads = t.threads[d];
+                ppt = t.ppt[d];
+                baseMs = t.tileMs[d];
+            }
+            const fn2::JitKernel* k[4] = {};
+            for( int v = 1; v < 4; v++ )
+            {
+                k[v] = fn2::GetJitKernel( prog, v );
+                if( !k[v] && fn2::JitVariantState( prog, v ) == 0 ) return;   // still compiling
+            }
+            {
+                std::lock_guard<std::mutex> lk( mu );
+                Tree& t = *trees[treeId];
+                if( t.unrollStage[d] != 1 || t.tuning[d] ) return;
+                t.tuning[d] = true;
+            }
+            int bestV = 0;
+            float bestMs = baseMs;
+            {
+                std::lock_guard<std::mutex> lk( tuneMu );
+                GenArgs a{};
+                int64_t n = 0;
+                if( TuneArgs( prog, d, a, n ) )
+                    for( int v = 1; v < 4; v++ )
+                    {
+                        if( !k[v] ) continue;
+                        float ms = TimeTile( prog, a, n, k[v], threads, ppt );
+                        if( ms >= 0 && ms < bestMs * 0.97f )
+                        {
+                            bestMs = ms;
+                            bestV = v;
+                        }
+                    }
+            }
+            std::lock_guard<std::mutex> lk( mu );
+            Tree& t = *trees[treeId];
+            t.variant[d] = bestV;
+            t.unrollStage[d] = 2;
+            t.tuning[d] = false;
+            r->variant = bestV;
+        }
+
+        // The tuner's request: a 2^20-point grid of the program in `dims`
+        // dimensions, writing into the tuning arena
+        bool TuneArgs( const fn2::DeviceProgram* prog, int dims, GenArgs& a, int64_t& n )
+        {
             fn2GenSpec spec{};
             spec.kind = FN2_SOURCE_GRID;
             spec.dims = dims;
             int side = dims == 2 ? 1024 : dims == 3 ? 128 : 32;
-            for( int a = 0; a < dims; a++ )
+            for( int ax = 0; ax < dims; ax++ )
             {
-                spec.count[a] = side;
-                spec.step[a] = 1.0f;
+                spec.count[ax] = side;
+                spec.step[ax] = 1.0f;
             }
             if( dims == 3 ) spec.count[2] = kTunePoints / ( side * side );
-            GenArgs a{};
-            int64_t n = 0;
             std::string err;
             if( !fn2::FillSpecArgs( spec, a, n, err ) )
             {
                 fn2::SetError( err );
-                return -1;
+                return false;
             }
             a.ins = prog->dIns;
             a.konst = prog->dKonst;
@@ -500,51 +654,48 @@ namespace
             a.out = tuneOut;
             a.minmax = tuneKeys;
             a.rangeStride = fn2::kRangeStrideCompact;
-            int best = 1;
-            float bestMs = 1e30f;
-            std::vector<int32_t> bj;
-            std::vector<uint32_t> bb;
-            size_t tb = (size_t)kTunePoints / kThreads + 1;
-            for( int ppt : kPptCandidates )
+            return true;
+        }
+
+        // Best of two timed runs (after one that warms caches) of the tuner's
+        // request at one tile, on the specialised batch kernel `jit` or the
+        // interpreter; -1 on a HIP error. Called with tuneMu held.
+        float TimeTile( const fn2::DeviceProgram* prog, GenArgs a, int64_t n, const fn2::JitKernel* jit, int threads, int ppt )
+        {
+            a.ppt = ppt;
+            const int64_t tile = (int64_t)threads * ppt;
+            const int nb = (int)( ( n + tile - 1 ) / tile );
+            std::vector<int32_t> bj( nb, 0 );
+            std::vector<uint32_t> bb( nb );
+            for( int b = 0; b < nb; b++ ) bb[b] = (uint32_t)( b * tile );
+            // the table holds kTunePoints / 64 + 1 entries per column
+            const size_t tb = (size_t)kTunePoints / kThreadCandidates[0] + 1;
+            int32_t* dBJ = tuneBJ;
+            uint32_t* dBB = (uint32_t*)( tuneBJ + tb );
+            if( !fn2::HipOk( fn2::UploadSync( tuneJob, &a, sizeof( a ) ), "tune upload" ) ||
+                !fn2::HipOk( fn2::UploadSync( dBJ, bj.data(), nb * sizeof( int32_t ) ), "tune upload" ) ||
+                !fn2::HipOk( fn2::UploadSync( dBB, bb.data(), nb * sizeof( uint32_t ) ), "tune upload" ) )
+                return -1.0f;
+            float ms = 1e30f;
+            for( int rep = 0; rep < 3; rep++ )
             {
-                a.ppt = ppt;
-                const int64_t tile = (int64_t)kThreads * ppt;
-                int nb = (int)( ( n + tile - 1 ) / tile );
-                bj.assign( nb, 0 );
-                bb.resize( nb );
-                for( int b = 0; b < nb; b++ ) bb[b] = (uint32_t)( b * tile );
-                int32_t* dBJ = tuneBJ;
-                uint32_t* dBB = (uint32_t*)( tuneBJ + tb );
-                if( !fn2::HipOk( fn2::UploadSync( tuneJob, &a, sizeof( a ) ), "tune upload" ) ||
-                    !fn2::HipOk( fn2::UploadSync( dBJ, bj.data(), nb * sizeof( int32_t ) ), "tune upload" ) ||
-                    !fn2::HipOk( fn2::UploadSync( dBB, bb.data(), nb * sizeof( uint32_t ) ), "tune upload" ) )
-                    return -1;
-                float ms = 1e30f;
-                for( int rep = 0; rep < 3; rep++ )
-                {
-                    const fn2::GenArgs* jobs = tuneJob;
-                    const int32_t* bjp = dBJ;
-                    const uint32_t* bbp = dBB;
-                    void* params[] = { (void*)&jobs, (void*)&bjp, (void*)&bbp };
-                    if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, fn2::kRangeStrideCompact, tuneStream ), "tune launch" ) ||
-                        !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
-                        !fn2::HipOk( jit ? hipModuleLaunchKernel( jit->batch, (unsigned)nb, 1, 1, kThreads, 1, 1, 0, tuneStream, params, nullptr )
-                                         : (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ),
-                                     "tune launch" ) ||
-                        !fn2::HipOk( hipEventRecord( tuneEv1, tuneStream ), "tune event" ) ||
-                        !fn2::HipOk( hipEventSynchronize( tuneEv1 ), "tune run" ) )
-                        return -1;
-                    float t = 0;
-                    if( !fn2::HipOk( hipEventElapsedTime( &t, tuneEv0, tuneEv1 ), "tune timing" ) ) return -1;
-                    if( rep > 0 ) ms = std::min( ms, t );   // first run warms caches
-                }
-                if( ms < bestMs * 0.97f )
-                {
-                    bestMs = ms;
-                    best = ppt;
-                }
+                const fn2::GenArgs* jobs = tuneJob;
+                const int32_t* bjp = dBJ;
+                const uint32_t* bbp = dBB;
+                void* params[] = { (void*)&jobs, (void*)&bjp, (void*)&bbp };
+                if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, fn2::kRangeStrideCompact, tuneStream ), "tune launch" ) ||
+                    !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
+                    !fn2::HipOk( jit ? hipModuleLaunchKernel( jit->batch, (unsigned)nb, 1, 1, (unsigned)threads, 1, 1, 0, tuneStream, params, nullptr )
+                                     : (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ),
+                                 "tune launch" ) ||
+                    !fn2::HipOk( hipEventRecord( tuneEv1, tuneStream ), "tune event" ) ||
+                    !fn2::HipOk( hipEventSynchronize( tuneEv1 ), "tune run" ) )
+                    return -1.0f;
+                float t = 0;
+                if( !fn2::HipOk( hipEventElapsedTime( &t, tuneEv0, tuneEv1 ), "tune timing" ) ) return -1.0f;
+                if( rep > 0 ) ms = std::min( ms, t );   // first run warms caches
             }
-            return best;
+            return ms;
         }
 
         std::mutex tuneMu;
@@ -568,6 +719,10 @@ namespace
         int64_t batches = 0;
         int64_t completed = 0;
         int64_t deviceBytes = 0;
+        // persistent mode: work-queue launches, those in which waves pulled
+        // items beyond the first round, and the pulls (counter increments)
+        int64_t queueLaunches = 0, queuePulledLaunches = 0, queuePulls = 0;
+        bool tuneUnroll = false;
         int maxBlocks = 0;
         int fillBlocks = 512;   // blocks that fill the GPU twice over (2 per CU)
         int cus = 256;          // compute units of the device
@@ -606,17 +761,27 @@ namespace
         {
             // jobs sorted by kernel so that each kernel's blocks are contiguous
             std::vector<Request*> order( s.reqs );
-            for( Request* r : order ) r->jit = fn2::GetJitKernel( r->prog );
-            std::stable_sort( order.begin(), order.end(),
-                              []( Request* a, Request* b ) { return KernelKey( a->prog, a->jit ) < KernelKey( b->prog, b->jit ); } );
+            for( Request* r : order )
+            {
+                r->jit = fn2::GetJitKernel( r->prog, r->variant );
+                if( !r->jit && r->variant ) r->jit = fn2::GetJitKernel( r->prog );
+                // threads per block: the tuned tile's on a specialised batch
+                // kernel; the interpreter's, and the work-queue kernels', are fixed
+                if( !r->jit || mode == FN2_SERVE_PERSISTENT ) r->threads = kThreads;
+            }
+            std::stable_sort( order.begin(), order.end(), []( Request* a, Request* b ) {
+                const int64_t ka = KernelKey( a->prog, a->jit ), kb = KernelKey( b->prog, b->jit );
+                return ka != kb ? ka < kb : a->threads < b->threads;
+            } );
             GenArgs* hJobs = (GenArgs*)s.hTables;
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
             size_t bjBytes = AlignUp( sizeof( int32_t ) * maxBlocks, 256 );
             int32_t* hBJ = (int32_t*)( s.hTables + jobsBytes );
             uint32_t* hBB = (uint32_t*)( s.hTables + jobsBytes + bjBytes );
-            // (kernel key, blocks, register-file size) per launch; the register
-            // file (interpreter launches only) sizes the launch's LDS
-            std::vector<std::array<int64_t, 3>> shape;
+            // (kernel key, blocks, register-file size, threads per block) per
+            // launch; the register file (interpreter launches only) sizes the
+            // launch's LDS
+            std::vector<std::array<int64_t, 4>> shape;
             std::vector<const fn2::JitKernel*> kernels;   // per entry of shape
             // A batch too small to fill the GPU at the tuned points per lane
             // runs with fewer points per lane (more blocks): the tuned value
@@ -635,7 +800,7 @@ namespace
                 a.minmax = s.dKeys + (size_t)jobRangeInts * j;
                 a.rangeStride = jobStride;
                 hJobs[j] = a;
-                const int64_t tile = (int64_t)kThreads * ppt;
+                const int64_t tile = (int64_t)r->threads * ppt;
                 int rb = (int)( ( r->points + tile - 1 ) / tile );
                 for( int b = 0; b < rb; b++ )
                 {
@@ -644,9 +809,9 @@ namespace
                 }
                 int64_t key = KernelKey( r->prog, r->jit );
                 const int64_t regs = r->jit ? 0 : r->prog->regs;
-                if( shape.empty() || shape.back()[0] != key )
+                if( shape.empty() || shape.back()[0] != key || shape.back()[3] != r->threads )
                 {
-                    shape.push_back( { key, rb, regs } );
+                    shape.push_back( { key, rb, regs, r->threads } );
                     kernels.push_back( r->jit );
                 }
                 else
@@ -673,7 +838,7 @@ namespace
                 {
                     shape.erase( shape.begin() + 1, shape.begin() + ni );
                     kernels.erase( kernels.begin() + 1, kernels.begin() + ni );
-                    shape[0] = { dm * 16 + fm, blocks, regs };
+                    shape[0] = { dm * 16 + fm, blocks, regs, kThreads };
                 }
             }
             const int nJobs = (int)order.size();
@@ -702,6 +867,7 @@ namespace
                 if( !fn2::HipOk( hipGraphLaunch( s.exec, s.stream ), "graph launch" ) ) return false;
             }
             else if( !Enqueue( s, shape, kernels, nJobs, nb ) ) return false;
+            s.queueLaunches = mode == FN2_SERVE_PERSISTENT ? (int)shape.size() : 0;
             if( !Copies( s, nJobs ) ) return false;
             if( !fn2::HipOk( hipEventRecord( s.done, s.stream ), "serve event" ) ) return false;
             s.reqs.swap( order );
@@ -711,7 +877,7 @@ namespace
         // Stream work of one batch that depends only on its shape (recorded
         // once per shape in graph mode): table upload, range-key reset, one
         // batch launch per kernel.
-        bool Enqueue( Slot& s, const std::vector<std::array<int64_t, 3>>& shape, const std::vector<const fn2::JitKernel*>& kernels,
+        bool Enqueue( Slot& s, const std::vector<std::array<int64_t, 4>>& shape, const std::vector<const fn2::JitKernel*>& kernels,
                       int nJobs, int nb )
         {
             size_t jobsBytes = AlignUp( sizeof( GenArgs ) * kMaxJobs, 256 );
@@ -749,7 +915,7 @@ namespace
                 {
                     const fn2::GenArgs* jobs = s.dJobs;
                     void* params[] = { (void*)&jobs, (void*)&bj, (void*)&bb };
-                    if( !fn2::HipOk( hipModuleLaunchKernel( kernels[gi]->batch, (unsigned)g[1], 1, 1, kThreads, 1, 1, 0, s.stream, params, nullptr ),
+                    if( !fn2::HipOk( hipModuleLaunchKernel( kernels[gi]->batch, (unsigned)g[1], 1, 1, (unsigned)g[3], 1, 1, 0, s.stream, params, nullptr ),
                                      "specialised batch launch" ) )
                         return false;
                 }
@@ -786,6 +952,12 @@ namespace
             }
             if( stagedEnd && !fn2::HipOk( hipMemcpyAsync( s.hOut, s.dOut, stagedEnd * 4, hipMemcpyDeviceToHost, s.stream ), "copy outputs" ) ) return false;
             if( !fn2::HipOk( hipMemcpyAsync( s.hKeys, s.dKeys, jobRangeInts * sizeof( int32_t ) * nJobs, hipMemcpyDeviceToHost, s.stream ), "copy ranges" ) ) return false;
+            // the work-queue counters: how many pulls each launch made
+            if( s.queueLaunches &&
+                !fn2::HipOk( hipMemcpyAsync( s.hKeys + (size_t)jobRangeInts * kMaxJobs, s.dCounter,
+                                             sizeof( uint32_t ) * fn2::kQueueCounterInts * s.queueLaunches, hipMemcpyDeviceToHost, s.stream ),
+                             "copy queue counters" ) )
+                return false;
             return true;
         }
 
@@ -849,6 +1021,19 @@ namespace
                             r->mm[1] = -__builtin_inff();
                         }
                     }
+                    int64_t pulled = 0, pulls = 0;
+                    const uint32_t* qc = (const uint32_t*)( s.hKeys + (size_t)jobRangeInts * kMaxJobs );
+                    for( int l = 0; l < s.queueLaunches; l++ )
+                    {
+                        int64_t n = 0;
+                        for( int c = 0; c < fn2::kQueueCounters; c++ ) n += qc[l * fn2::kQueueCounterInts + c * fn2::kQueueCounterStride];
+                        pulls += n;
+                        pulled += n > 0;
+                    }
+                    std::lock_guard<std::mutex> lk( mu );
+                    queueLaunches += s.queueLaunches;
+                    queuePulledLaunches += pulled;
+                    queuePulls += pulls;
                 }
                 auto now = Clock::now();
                 {
@@ -943,6 +1128,26 @@ FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims )
 {
     if( !engine ) return -1;
     return static_cast<Engine*>( engine )->TreePpt( treeId, dims );
+}
+
+FASTNOISE_API int fn2ServeTuneOctaveUnroll( void* engine, int enable )
+{
+    if( !engine ) return 1;
+    static_cast<Engine*>( engine )->SetTuneUnroll( enable != 0 );
+    return 0;
+}
+
+FASTNOISE_API int fn2ServeQueueStats( void* engine, int64_t* launches, int64_t* pulledLaunches, int64_t* pulls )
+{
+    if( !engine ) return 1;
+    static_cast<Engine*>( engine )->QueueStats( launches, pulledLaunches, pulls );
+    return 0;
+}
+
+FASTNOISE_API int fn2ServeTreeTile( void* engine, int treeId, int dims, int* threadsPerBlock, int* pointsPerLane, int* octaveUnroll )
+{
+    if( !engine ) return -1;
+    return static_cast<Engine*>( engine )->TreeTile( treeId, dims, threadsPerBlock, pointsPerLane, octaveUnroll );
 }
 
 FASTNOISE_API int fn2ServeReplay( void* engine, int count, const int* treeIds, const fn2GenSpec* specs, const double* arrivalSec,

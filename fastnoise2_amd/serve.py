@@ -97,6 +97,27 @@ class ServeEngine:
         """Points per lane chosen for the tree's batched launches (0 = not tuned yet)."""
         return self.lib.fn2ServeTreeTuning(self.h, tree, dims)
 
+    def tune_octave_unroll(self, enable=True):
+        """Lets the tuner time the octave-unroll variants of trees with fused
+        fractals (off by default)."""
+        self.lib.fn2ServeTuneOctaveUnroll(self.h, 1 if enable else 0)
+
+    def queue_stats(self):
+        """Persistent mode: (work-queue launches, launches whose waves pulled
+        items beyond their first, pulls made)."""
+        v = [ctypes.c_int64() for _ in range(3)]
+        self.lib.fn2ServeQueueStats(self.h, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def tree_tile(self, tree, dims):
+        """(threads per block, points per lane, octave unroll) the tuner chose
+        for the tree in `dims` dimensions, or None when not tuned yet."""
+        t, p, u = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rc = self.lib.fn2ServeTreeTile(self.h, tree, dims, ctypes.byref(t), ctypes.byref(p), ctypes.byref(u))
+        if rc < 0:
+            raise ValueError("bad tree or dimensionality")
+        return None if rc else (t.value, p.value, u.value)
+
     def stats(self):
         b, c, p = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
         self.lib.fn2ServeStats(self.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(p))

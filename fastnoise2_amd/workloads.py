@@ -152,6 +152,22 @@ def mixed_trees(seed=5, count=8):
     return [random_tree(rng, int(d)).encode() for d in depths]
 
 
+# (FP32 FLOPs, vector ALU operations) per point of each mixed tree in 2D, 3D
+# and 4D, counted by the op-counting oracle on a 32^2 / 16^3 / 8^4 grid at the
+# origin (the trees branch on nothing, so the counts do not depend on position)
+MIXED_TREE_OPS = [
+    {2: (590.0, 936.0), 3: (592.0, 937.0), 4: (2174.0, 3446.0)},
+    {2: (288.0, 607.0), 3: (381.0, 1059.0), 4: (653.0, 2239.0)},
+    {2: (785.0, 1326.0), 3: (2398.0, 4171.0), 4: (8417.0, 14258.0)},
+    {2: (751.0, 1333.0), 3: (2114.0, 3643.0), 4: (6971.0, 11024.0)},
+    {2: (389.0, 1101.0), 3: (391.0, 1102.0), 4: (672.0, 2398.0)},
+    {2: (6027.0, 10739.0), 3: (10071.0, 19003.0), 4: (36019.0, 65490.0)},
+    {2: (4160.0, 8807.0), 3: (5332.0, 10688.0), 4: (15361.0, 28849.0)},
+    {2: (12865.0, 20614.0), 3: (24583.0, 41557.0), 4: (85815.0, 137879.0)},
+]
+MIXED_OPS_SAMPLE = {2: [32, 32], 3: [16, 16, 16], 4: [8, 8, 8, 8]}
+
+
 def mixed_trace(n=4000, seed=77, burst_ms=10.0, burst_rate=10000.0, off_mean_ms=40.0, ntrees=8):
     """Bursty requests over the mixed trees: on/off arrivals (bursts of
     `burst_ms` at `burst_rate` requests/s, exponential off periods), uniform
@@ -193,7 +209,10 @@ def bench_workloads():
     `ops` is (FP32 FLOPs with an fma counted as 2, vector ALU operations) per
     point, counted by the op-counting build of the oracle over `ops_sample`
     (a corner of the workload's own grid, or a coarse grid over the position
-    cube); the counts are committed here and recounted by the CPU tests."""
+    cube); the counts are committed here and recounted by the CPU tests.
+    `pruned`: the tree's CellularDistance takes the pruned cell walk, which
+    leaves out cells the reference's fixed 27-cell walk (and so the count)
+    includes."""
     return {
         "fbm_simplex_2d_grid_4096x4096": dict(
             tree=fbm_simplex_2d, kind="grid", dims=2, offset=[-2048, -2048], count=[4096, 4096], seed=42,
@@ -206,10 +225,10 @@ def bench_workloads():
             ops=(69.0, 209.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
         "warped_cellular_3d_grid_256x256x256": dict(
             tree=warped_cellular_3d, kind="grid", dims=3, offset=[0, 0, 0], count=[256, 256, 256], seed=1337,
-            ops=(837.0, 1229.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32])),
+            ops=(837.0, 1229.0), ops_sample=dict(dims=3, offset=[0, 0, 0], count=[32, 32, 32]), pruned=True),
         "warped_cellular_positions_16777216": dict(
             tree=warped_cellular_3d, kind="positions", dims=3, count=[1 << 24], seed=1337,
-            ops=(837.0, 1229.0),
+            ops=(837.0, 1229.0), pruned=True,
             ops_sample=dict(dims=3, offset=[-1000, -1000, -1000], count=[16, 16, 16], step=[125, 125, 125])),
         "serving_tree_3d_grid_256x256x256": dict(
             tree=serving_tree, kind="grid", dims=3, offset=[0, 0, 0], count=[256, 256, 256], seed=1337,

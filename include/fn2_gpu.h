@@ -127,6 +127,17 @@ FASTNOISE_API int fn2JitWait( void );
  * compiler log), 3 kept on the interpreter (over the instruction limit),
  * -2 bad arguments. */
 FASTNOISE_API int fn2GetJitState( const void* node, int dims );
+/* The same for a variant of the specialised kernel (same results, other
+ * compile-time choices): bits 0-1 the fused fractals' octave-loop unroll
+ * (0 the compiler's choice, 1 none, 2 by two, 3 by four), bit 2 the shared
+ * far-cell walk that position-array requests of 3D CellularDistance trees use. */
+#define FN2_JIT_VARIANT_SHARED_CELL_WALK 4
+FASTNOISE_API int fn2GetJitVariantState( const void* node, int dims, int variant );
+/* Selects the octave-unroll variant (0..3 as above) that single calls of a
+ * tree in `dims` dimensions run once it is compiled (the default kernel
+ * meanwhile); the serving engine's tuner picks its own. Same results for
+ * every variant. 0 on success. */
+FASTNOISE_API int fn2SetJitVariant( const void* node, int dims, int variant );
 
 /* Compiled-program facts for a tree evaluated in `dims` dimensions. */
 FASTNOISE_API int fn2GetProgramInfo( const void* node, int dims, int* instructions, int* registers, int* dimMask, int* featureMask );
@@ -176,11 +187,31 @@ FASTNOISE_API int fn2ServePoll( void* engine, int64_t requestId, float* outputMi
 /* Engine counters: batches launched, requests completed, device bytes held. */
 FASTNOISE_API int fn2ServeStats( void* engine, int64_t* batches, int64_t* completed, int64_t* peakBytes );
 
+/* Lets the tuner also time the octave-unroll variants of trees with fused
+ * fractals (compiled in the background once the tree's tile is tuned on its
+ * specialised kernel; fn2ServeTreeTile reports the choice). Off by default:
+ * on the mixed trees it found 2-8 % on some trees and no throughput overall,
+ * for three more compilations per tree and dimensionality. 0 on success. */
+FASTNOISE_API int fn2ServeTuneOctaveUnroll( void* engine, int enable );
+
+/* Persistent mode: work-queue launches so far, how many of them had more
+ * items than their resident waves (so that waves pulled further items from
+ * the launch's counters), and the pulls made (counter increments). */
+FASTNOISE_API int fn2ServeQueueStats( void* engine, int64_t* launches, int64_t* pulledLaunches, int64_t* pulls );
+
 /* Points per lane the engine's tuner chose for a tree's batched launches in
  * `dims` dimensions (1, 2, 4 or 8; 0 = not tuned yet, -1 = bad arguments). The
  * tuner times a 2^20-point request of the tree at each candidate on the
  * first request of that tree and dimensionality. */
 FASTNOISE_API int fn2ServeTreeTuning( void* engine, int treeId, int dims );
+/* The whole tile the tuner chose for a tree in `dims` dimensions: threads per
+ * block of its specialised batch kernel (64, 128 or 256; the interpreter and
+ * the persistent mode use 256), points per lane, and the unroll factor of
+ * its fused fractals' octave loops (0 = the compiler's choice, else 1, 2 or
+ * 4; tried once the variants have compiled in the background). Every tile
+ * gives the same bits. Returns 0, 1 when not tuned yet, -1 on bad
+ * arguments. */
+FASTNOISE_API int fn2ServeTreeTile( void* engine, int treeId, int dims, int* threadsPerBlock, int* pointsPerLane, int* octaveUnroll );
 
 /* Replays a request trace against the engine from the calling thread:
  * request i (tree treeIds[i], spec specs[i]) is submitted at arrivalSec[i]

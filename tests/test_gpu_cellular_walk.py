@@ -1,6 +1,9 @@
 """CellularDistance with squared distances, in a tree's specialised kernel,
-visits cells centre-first and leaves out, per wave, cells that cannot reach
-the kept distance slots; grid requests of such trees are walked in bricks.
+visits cells near side first and leaves out, per wave, cells that cannot
+reach the kept distance slots; grid requests of such trees are walked in
+bricks. Position arrays of 3D trees with one or two kept slots run a variant
+of the kernel in which a wave whose lanes need many different far cells
+shares them out through an LDS queue instead.
 These cases stress that walk against the oracle's full walk, bit for bit:
 coherent and ragged grids (where whole waves skip cells) and scattered
 positions (where they rarely do), pairs of distance indices, jitter values
@@ -56,13 +59,16 @@ def test_scattered_positions_match_oracle(gpu, dims):
     pos[:, :1000] = rng.normal(0.0, 0.3, (dims, 1000)).astype(np.float32) * 9.0
     pos[:, 1000:1500] = (np.round(pos[:, 1000:1500] / 9.0) * 9.0).astype(np.float32)
     pos[:, 1500:2000] = ((np.floor(pos[:, 1500:2000] / 9.0) + 0.5) * 9.0).astype(np.float32)
-    for jitter in (1.0, -0.7, 2.5):
-        spec = tree(jitter, 0, 1, "Euclidean")
+    for jitter, i0, i1 in ((1.0, 0, 1), (-0.7, 0, 1), (2.5, 0, 1), (1.0, 0, 0), (0.8, 1, 0)):
+        spec = tree(jitter, i0, i1, "Euclidean")
         node = fn.Node.from_encoded(spec.encode())
         out, mm = node.gen_position_array(list(pos), seed=77)
         assert node.jit_state(dims) == 1, fn.last_error()
+        # 3D position arrays run the shared far-cell walk (one or two slots)
+        shared = node.jit_state(dims, fn.JIT_VARIANT_SHARED_CELL_WALK)
+        assert shared == (1 if dims == 3 else -1), (shared, fn.last_error())
         ref, ref_mm = oracle.Tree(spec.encode()).positions(list(pos), seed=77)
-        assert_bitexact(out, ref, "%dD scattered jitter %g" % (dims, jitter))
+        assert_bitexact(out, ref, "%dD scattered jitter %g indices %d,%d" % (dims, jitter, i0, i1))
         assert mm == ref_mm
 
 

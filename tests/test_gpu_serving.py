@@ -184,25 +184,35 @@ def test_graph_replays_with_varying_request_sizes(gpu):
 
 
 def test_persistent_queue_with_more_tiles_than_resident_blocks(gpu):
-    # one batch of large requests of every tree: its tiles (256 points each at
-    # one point per lane) outnumber the resident blocks many times over, so
-    # every block pulls many tiles of different trees from the counter
+    # the interpreter's work-queue kernel over batches of large requests of
+    # every tree: a batch of the four or more that follow the first holds
+    # at least 8.4 M points, at least 4096 tiles at any points per lane,
+    # while at most 2048 blocks are resident (8 per CU), so waves pull items
+    # beyond their first from the launch's counters; the engine's counter
+    # read-back shows the pulls happened
     nodes = [fn.Node.from_encoded(t.encode()) for t in TREES]
-    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 22, mode=SERVE_PERSISTENT)
-    tids = [eng.add_tree(t.encode()) for t in TREES]
-    reqs = []
-    for k in range(2):
-        for t in range(len(TREES)):
-            c = 96 if k == 0 else 57
-            reqs.append((t, fn.grid_spec(3, [k * 3000.0, -float(t) * 100, 7.0], [c, c, c], seed=11 * t + k)))
-    ids = [eng.submit(tids[t], spec) for t, spec in reqs]
-    for (t, spec), rid in zip(reqs, ids):
-        out, mm, _ = eng.wait(rid)
-        ref, ref_mm = alone(nodes, t, spec)
-        assert_bitexact(out, ref, "persistent tree %d count %d" % (t, spec.count[0]))
-        assert mm == ref_mm
-    assert eng.stats()["batches"] < len(reqs)
-    eng.close()
+    old = fn.set_jit_mode(fn.JIT_OFF)
+    try:
+        eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 24, mode=SERVE_PERSISTENT)
+        tids = [eng.add_tree(t.encode()) for t in TREES]
+        reqs = []
+        for k in range(2):
+            for t in range(len(TREES)):
+                c = 128 if k == 0 else 57
+                reqs.append((t, fn.grid_spec(3, [k * 3000.0, -float(t) * 100, 7.0], [c, c, c], seed=11 * t + k)))
+        ids = [eng.submit(tids[t], spec) for t, spec in reqs]
+        for (t, spec), rid in zip(reqs, ids):
+            out, mm, _ = eng.wait(rid)
+            ref, ref_mm = alone(nodes, t, spec)
+            assert_bitexact(out, ref, "persistent tree %d count %d" % (t, spec.count[0]))
+            assert mm == ref_mm
+        assert eng.stats()["batches"] < len(reqs)
+        launches, pulled, pulls = eng.queue_stats()
+        print("work-queue launches %d, with pulls %d, pulls %d" % (launches, pulled, pulls))
+        assert launches >= 1 and pulled >= 1 and pulls > 0, (launches, pulled, pulls)
+        eng.close()
+    finally:
+        fn.set_jit_mode(old)
 
 
 @pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN, SERVE_PERSISTENT])
