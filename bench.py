@@ -53,6 +53,11 @@ HBM_BOUND = {"constant_2d_grid_16384x16384", "checkerboard_2d_grid_16384x16384"}
 SQ_COUNTERS = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES",
                "SQ_INSTS_VALU_TRANS_F32", "GRBM_GUI_ACTIVE"]
 COUNTER_PASSES = [["FETCH_SIZE"], ["WRITE_SIZE"] + SQ_COUNTERS]
+# where the wave-cycles go (an optional third pass: the bench goes on without
+# it if the counters are not there): alive, waiting on anything, waiting to
+# issue, issuing
+STALL_COUNTERS = ["SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_SCA",
+                  "SQ_ACTIVE_INST_MISC"]
 
 
 def make_spec(fn, W, wl, keep):
@@ -160,12 +165,12 @@ def _is_generator(kernel_name):
 
 
 def measure_counters(names):
-    """Hardware counters per launch of each workload's kernel, from two
-    rocprofv3 --pmc passes over one child run of all the workloads: HBM bytes
-    (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch, each scaled by what the
-    same counter reads for the 256 MiB copy, which moves 256 MiB each way) and
-    the instruction / busy counters. {} when rocprofv3 is missing or a pass
-    fails."""
+    """Hardware counters per launch of each workload's kernel, from
+    rocprofv3 --pmc passes over one child run of all the workloads each: HBM
+    bytes (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch, each scaled by
+    what the same counter reads for the 256 MiB copy, which moves 256 MiB each
+    way) with the instruction / busy counters, then the wave-cycle counters.
+    {} when rocprofv3 is missing or one of the first two passes fails."""
     import csv
     import shutil
     import subprocess
@@ -173,7 +178,8 @@ def measure_counters(names):
     if not shutil.which("rocprofv3"):
         return {}
     res = {n: {} for n in names}
-    for counters in COUNTER_PASSES:
+    for counters in COUNTER_PASSES + [STALL_COUNTERS]:
+        optional = counters is STALL_COUNTERS
         d = tempfile.mkdtemp(prefix="fn2_pmc_")
         cmd = ["timeout", "-s", "KILL", "240", "rocprofv3", "--pmc"] + counters + [
             "--kernel-trace", "-f", "csv", "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__),
@@ -199,6 +205,8 @@ def measure_counters(names):
         if len(order) != len(names) or not copies or len(gens) != sum(o["dispatches"] for o in order):
             progress("counter pass %s gave no usable rows (exit %d, %d workloads, %d generator dispatches); its output "
                      "ends with: %s" % (counters[0], p.returncode, len(order), len(gens), (p.stdout + p.stderr)[-1500:]))
+            if optional:
+                continue
             return {}
         at = 0
         for o in order:
@@ -280,6 +288,15 @@ def roofline_for(name, wl, n, kernel_ms, counters, kernel_name, interp_ms=None):
             "clock_GHz": round(cycles / kernel_s / 1e9, 3) if cycles else None,
             "counters": c,
         }
+        if c.get("SQ_WAVE_CYCLES"):
+            # shares of the waves' lifetime (wave-cycles): waiting on anything
+            # (a dependency, memory, a barrier), waiting for an issue slot,
+            # and issuing VALU / scalar instructions
+            wc = float(c["SQ_WAVE_CYCLES"])
+            r["hw"]["wave_cycles"] = {k: round(c[n] / wc, 4) for k, n in (
+                ("wait_any", "SQ_WAIT_ANY"), ("wait_inst_any", "SQ_WAIT_INST_ANY"), ("active_inst_any", "SQ_ACTIVE_INST_ANY"),
+                ("active_inst_valu", "SQ_ACTIVE_INST_VALU"), ("active_inst_scalar", "SQ_ACTIVE_INST_SCA"),
+                ("active_inst_misc", "SQ_ACTIVE_INST_MISC")) if n in c}
     return r
 
 

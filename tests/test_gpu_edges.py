@@ -127,3 +127,26 @@ def test_oversized_request_is_rejected(gpu):
     L.fnGenTileable2D(node.handle, sentinel.ctypes.data_as(fp), 1 << 30, 1 << 30, 1, 1, 1, mm.ctypes.data_as(fp))
     assert (sentinel == 7.0).all()
     assert "2^31" in fn.last_error()
+
+
+@pytest.mark.parametrize("kind", ["FractalFBm", "FractalRidged"])
+@pytest.mark.parametrize("ws", [0.0, -0.0, 0.6])
+@pytest.mark.parametrize("jit", [fn.JIT_OFF, fn.JIT_SYNC], ids=["interpreter", "specialised"])
+def test_fused_fractal_special_positions(gpu, kind, ws, jit):
+    # the fused octave loop's weighted-strength step: with weighted strength
+    # zero it only has to turn the amplitude into NaN when an octave's value
+    # is not finite; infinite, NaN and huge positions reach that case
+    spec = S(kind, source=S("Simplex"), octaves=4, gain=0.55, weighted_strength=ws, lacunarity=2.1)
+    node = fn.Node.from_encoded(spec.encode())
+    xs = np.array([0.0, -0.0, 1e-30, 1e6, -3.0e7, 3.0e9, np.inf, -np.inf, np.nan, 2.5, 1e38, -1e38], dtype=np.float32)
+    ys = np.array([0.0, 1.0, -1.0, 5.5, 0.25, 1.0, 0.0, 1.0, 2.0, np.nan, -1e38, 7.0], dtype=np.float32)
+    old = fn.set_jit_mode(jit)
+    try:
+        out, mm = node.gen_position_array([xs, ys], seed=11)
+        if jit == fn.JIT_SYNC:
+            assert node.jit_state(2) == 1, fn.last_error()
+    finally:
+        fn.set_jit_mode(old)
+    ref, ref_mm = oracle.Tree(spec.encode()).positions([xs, ys], seed=11)
+    assert_bitexact(out, ref, "%s ws %g special positions" % (kind, ws))
+    assert mm == ref_mm or (np.isnan(mm[0]) and np.isnan(ref_mm[0]))
