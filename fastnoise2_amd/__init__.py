@@ -284,6 +284,15 @@ class Node:
         if lib().fn2SetJitVariant(self.handle, dims, variant):
             raise ValueError("fn2SetJitVariant failed: " + last_error())
 
+    def jit_kernel_info(self, dims, variant=0):
+        """(VGPRs, scratch bytes per lane, resident blocks per CU) of the loaded
+        specialised kernel, or None when it is not loaded."""
+        v, sc, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rc = lib().fn2GetJitKernelInfo(self.handle, dims, variant, ctypes.byref(v), ctypes.byref(sc), ctypes.byref(b))
+        if rc < 0:
+            raise ValueError("fn2GetJitKernelInfo failed: " + last_error())
+        return None if rc else (v.value, sc.value, b.value)
+
     def jit_source(self, dims):
         """HIP source of the tree's specialised kernel."""
         n = lib().fn2JitSource(self.handle, dims, None, 0)

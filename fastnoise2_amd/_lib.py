@@ -110,6 +110,7 @@ GPU_EXT = {
     "fn2GetJitState": (i, [vp, i]),
     "fn2GetJitVariantState": (i, [vp, i, i]),
     "fn2SetJitVariant": (i, [vp, i, i]),
+    "fn2GetJitKernelInfo": (i, [vp, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
     "fn2GetProgramInfo": (i, [vp, i, c_int_p, c_int_p, c_int_p, c_int_p]),
     "fn2ServeCreate": (vp, [ctypes.c_size_t, i, i]),
     "fn2ServeDestroy": (None, [vp]),

@@ -538,6 +538,7 @@ namespace fn2i
         }
         else if( a.kind == fn2::SRC_TILEABLE )
         {
+            idx += a.base;
             uint32_t yi = fastDiv( a.div[0], idx );
             uint32_t xi = idx - yi * a.div[0].d;
             // per-column and per-row torus coordinates, tabulated by the host
@@ -548,7 +549,7 @@ namespace fn2i
         }
         else
         {
-            uint32_t rem = idx;
+            uint32_t rem = idx + a.base;
             for( int ax = 0; ax < a.dims - 1; ax++ )
             {
                 uint32_t q = fastDiv( a.div[ax], rem );
@@ -574,6 +575,7 @@ namespace fn2i
         }
         else if( D == 4 && a.kind == fn2::SRC_TILEABLE )
         {
+            idx += a.base;
             uint32_t yi = fastDiv( a.div[0], idx );
             uint32_t xi = idx - yi * a.div[0].d;
             p[0] = a.pos[0][xi];
@@ -583,7 +585,7 @@ namespace fn2i
         }
         else
         {
-            uint32_t rem = idx;
+            uint32_t rem = idx + a.base;
 #pragma unroll
             for( int ax = 0; ax < D - 1; ax++ )
             {

@@ -211,6 +211,12 @@ namespace fn2
                 (void)hipGetLastError();
                 k.queueBlocksPerCU = 0;
             }
+            if( hipFuncGetAttribute( &k.singleRegs, HIP_FUNC_ATTRIBUTE_NUM_REGS, k.single ) != hipSuccess ||
+                hipFuncGetAttribute( &k.singleScratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, k.single ) != hipSuccess )
+            {
+                (void)hipGetLastError();
+                k.singleRegs = k.singleScratch = -1;
+            }
             return true;
         }
 
@@ -500,6 +506,21 @@ FASTNOISE_API int fn2SetJitVariant( const void* node, int dims, int variant )
     const fn2::DeviceProgram* dp = nullptr;
     if( !fn2::EnsureDevice() || !fn2::CompileForDevice( static_cast<const FastNoise::Generator*>( node ), dims, dp, nullptr ) ) return 1;
     dp->variant.store( variant, std::memory_order_relaxed );
+    return 0;
+}
+
+FASTNOISE_API int fn2GetJitKernelInfo( const void* node, int dims, int variant, int* vgprs, int* scratchBytes, int* blocksPerCU )
+{
+    fn2::ClearError();
+    if( !node || variant < 0 || variant > ( fn2::kJitUnrollMask | fn2::kJitSharedWalk ) ) return -1;
+    const fn2::DeviceProgram* dp = nullptr;
+    if( !fn2::EnsureDevice() || !fn2::CompileForDevice( static_cast<const FastNoise::Generator*>( node ), dims, dp, nullptr ) ) return -1;
+    if( fn2::Manager().State( dp, variant, nullptr ) != 1 ) return 1;
+    const fn2::JitKernel* k = fn2::GetJitKernel( dp, variant );
+    if( !k ) return 1;
+    if( vgprs ) *vgprs = k->singleRegs;
+    if( scratchBytes ) *scratchBytes = k->singleScratch;
+    if( blocksPerCU ) *blocksPerCU = k->blocksPerCU;
     return 0;
 }
 

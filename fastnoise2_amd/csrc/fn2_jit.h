@@ -15,6 +15,8 @@ namespace fn2
         hipFunction_t queue = nullptr;    // fn2_jit_queue( jobs, tileJob, tileBase, nTiles, counter )
         int blocksPerCU = 0;              // resident 256-thread blocks of `single` per CU
         int queueBlocksPerCU = 0;         // the same for `queue`
+        int singleRegs = 0;               // VGPRs per lane of `single`, as the runtime reports them
+        int singleScratch = 0;            // private (scratch) bytes per lane of `single`
     };
 
     // Variants of a program's specialised kernel: compile-time choices that

@@ -70,5 +70,9 @@ namespace fn2
         int32_t ppt;             // batched kernel: points per lane of each block of this job
         int32_t regs;            // registers the program uses (the interpreter's file holds as many)
         int32_t rangeStride;     // ints between the slots of the range record
+        // index of this job's first point in its request: a request split
+        // into several jobs evaluates points base + [0, total) of the whole
+        // grid or image (position arrays are offset by the host instead)
+        uint32_t base;
     };
 }

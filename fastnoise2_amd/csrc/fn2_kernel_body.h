@@ -17,6 +17,10 @@ namespace fn2k
     using namespace fn2d;
     using namespace fn2i;
 
+    // Output stores: each value is written once and never read back by the
+    // kernel, so they go out as streaming (non-temporal) stores
+    __device__ __forceinline__ void storeOut( float* p, float v ) { __builtin_nontemporal_store( v, p ); }
+
     // BLOCK: the most threads a block of the kernel has (its launch bound);
     // the block may have fewer, any multiple of 64.
     template<int BLOCK>
@@ -119,7 +123,7 @@ namespace fn2k
                 fresh.R[1] = fmadd( (float)(int32_t)y, a.step[1], a.offset[1] );
                 if constexpr( D == 3 ) fresh.R[2] = fmadd( (float)(int32_t)z, a.step[2], a.offset[2] );
                 const float v = prog( a, fresh.R );
-                a.out[( z * ny + y ) * nx + x] = v;
+                storeOut( a.out + ( z * ny + y ) * nx + x, v );
                 if( v == v )
                 {
                     mn = fminf( mn, v );
@@ -135,7 +139,7 @@ namespace fn2k
     {
         float mn = __builtin_inff(), mx = -__builtin_inff();
         auto emit = [&]( uint32_t i, float v ) {
-            a.out[i] = v;
+            storeOut( a.out + i, v );
             if( v == v )
             {
                 mn = fminf( mn, v );
@@ -187,7 +191,7 @@ namespace fn2k
     {
         float mn = __builtin_inff(), mx = -__builtin_inff();
         auto emit = [&]( uint32_t i, float v ) {
-            a.out[i] = v;
+            storeOut( a.out + i, v );
             if( v == v )
             {
                 mn = fminf( mn, v );
@@ -303,7 +307,7 @@ namespace fn2k
             const uint32_t base = tileBase[t] + ( w % kWaves ) * 64 + lane;
             const int ppt = a.ppt;
             auto emit = [&]( uint32_t i, float v ) {
-                a.out[i] = v;
+                storeOut( a.out + i, v );
                 if( v == v )
                 {
                     mn = fminf( mn, v );

@@ -131,6 +131,10 @@ namespace
         bool done = false;
         int status = 0;
         float mm[2] = { 0, 0 };
+        // a part of a request split across batches (its parent owns the id),
+        // and a split request's parts still running
+        Request* parent = nullptr;
+        int pending = 0;
     };
 
     struct Slot
@@ -275,6 +279,8 @@ namespace
                 if( s.done ) (void)hipEventDestroy( s.done );
                 if( s.stream ) (void)hipStreamDestroy( s.stream );
             }
+            for( Request* q : queue )
+                if( q->parent ) delete q;   // parts never launched
             for( auto& kv : requests ) delete kv.second;
             if( tuneStream ) (void)hipStreamSynchronize( tuneStream );
             if( tuneOut ) (void)hipFree( tuneOut );
@@ -344,11 +350,6 @@ namespace
             if( !fn2::FillSpecArgs( spec, r->args, r->points, err ) )
             {
                 fn2::SetError( err );
-                return -1;
-            }
-            if( r->points > capacity )
-            {
-                fn2::SetError( "request larger than the engine's batch capacity" );
                 return -1;
             }
             const fn2::DeviceProgram* prog = nullptr;
@@ -447,12 +448,36 @@ namespace
             r->args.nIns = (int32_t)prog->ins.size();
             r->args.resultReg = prog->resultReg;
             r->tSubmit = Clock::now();
+            // A request beyond a batch slot's capacity is split into parts of
+            // at most `capacity` points, each a job over points base +
+            // [0, part) of the whole grid or image, written to its own range
+            // of the destination. The request completes when every part has,
+            // with the parts' ranges merged; the parts go through the queue
+            // like requests, so batches of other requests interleave.
+            std::vector<Request*> parts;
+            if( r->points > capacity )
+            {
+                for( int64_t b = 0; b < r->points; b += capacity )
+                {
+                    Request* c = new Request( *r );
+                    c->parent = r.get();
+                    c->points = std::min<int64_t>( capacity, r->points - b );
+                    c->args.total = (int32_t)c->points;
+                    c->args.base = (uint32_t)b;
+                    c->hostOut = r->hostOut + b;
+                    parts.push_back( c );
+                }
+                r->pending = (int)parts.size();
+                r->mm[0] = __builtin_inff();
+                r->mm[1] = -__builtin_inff();
+            }
             Request* raw = r.release();
             {
                 std::lock_guard<std::mutex> lk( mu );
                 raw->id = nextId++;
                 requests[raw->id] = raw;
-                queue.push_back( raw );
+                if( parts.empty() ) queue.push_back( raw );
+                for( Request* c : parts ) queue.push_back( c );
             }
             cvWork.notify_one();
             return raw->id;
@@ -1042,6 +1067,21 @@ namespace
                     {
                         if( !ok ) r->status = 1;
                         r->tDone = now;
+                        if( Request* p = r->parent )
+                        {
+                            // a part: fold it into its request
+                            if( r->status ) p->status = 1;
+                            p->mm[0] = std::min( p->mm[0], r->mm[0] );
+                            p->mm[1] = std::max( p->mm[1], r->mm[1] );
+                            if( --p->pending == 0 )
+                            {
+                                p->tDone = now;
+                                p->done = true;
+                                completed++;
+                            }
+                            delete r;
+                            continue;
+                        }
                         r->done = true;
                         completed++;
                     }

@@ -138,6 +138,11 @@ FASTNOISE_API int fn2GetJitVariantState( const void* node, int dims, int variant
  * meanwhile); the serving engine's tuner picks its own. Same results for
  * every variant. 0 on success. */
 FASTNOISE_API int fn2SetJitVariant( const void* node, int dims, int variant );
+/* Resources of a loaded specialised kernel (single-call entry point): VGPRs
+ * per lane and scratch bytes per lane as the runtime reports them, and its
+ * resident 256-thread blocks per CU. 0, 1 when that variant is not loaded,
+ * -1 on bad arguments. */
+FASTNOISE_API int fn2GetJitKernelInfo( const void* node, int dims, int variant, int* vgprs, int* scratchBytes, int* blocksPerCU );
 
 /* Compiled-program facts for a tree evaluated in `dims` dimensions. */
 FASTNOISE_API int fn2GetProgramInfo( const void* node, int dims, int* instructions, int* registers, int* dimMask, int* featureMask );
@@ -169,8 +174,10 @@ FASTNOISE_API int fn2ServeAddTree( void* engine, const char* encodedTree );
  * positions are not accepted), output buffer of the spec's point count.
  * The output may be pageable host memory (staged through pinned memory),
  * pinned host memory (filled straight from the device) or device memory
- * (a device-to-device copy). Returns a request id (>= 0) or -1 when the
- * request can never fit the budget. The call does not block on the GPU. */
+ * (a device-to-device copy). A request larger than one batch
+ * (maxBatchPoints) runs as several parts in successive batches and completes
+ * when the last part has. Returns a request id (>= 0) or -1 on bad arguments.
+ * The call does not block on the GPU. */
 FASTNOISE_API int64_t fn2ServeSubmit( void* engine, int treeId, const fn2GenSpec* spec, float* hostOut );
 
 /* Waits for a request; fills its output buffer and min/max. latencyMs:

@@ -198,3 +198,16 @@ def test_persistent_specialised_queue_pulls(gpu, jit_sync):
     print("work-queue launches %d, with pulls %d, pulls %d" % (launches, pulled, pulls))
     assert pulled >= 1 and pulls > 0, (launches, pulled, pulls)
     eng.close()
+
+
+def test_jit_kernel_info(gpu, jit_sync):
+    # the runtime's view of a loaded specialised kernel: registers, scratch
+    # and residency; none before it is compiled
+    node = fn.Node.from_encoded(W.fbm_simplex_2d().encode())
+    assert node.jit_kernel_info(3) is None
+    node.gen_uniform_grid_2d(0.0, 0.0, 64, 64, seed=1)
+    regs, scratch, blocks = node.jit_kernel_info(2)
+    print("FBm specialised kernel: %d VGPRs, %d scratch bytes, %d blocks per CU" % (regs, scratch, blocks))
+    assert 0 < regs <= 512 and scratch >= 0 and 1 <= blocks <= 8
+    with pytest.raises(ValueError):
+        node.jit_kernel_info(2, variant=99)

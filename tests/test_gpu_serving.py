@@ -124,11 +124,15 @@ def test_graph_replays_stay_correct(gpu):
     eng.close()
 
 
-def test_admission_rejects_oversized_and_bad_requests(gpu):
+def test_admission_splits_oversized_and_rejects_bad_requests(gpu):
     eng = ServeEngine(hbm_budget_bytes=1 << 28, max_batch_points=1 << 12, mode=SERVE_BATCHED)
     tid = eng.add_tree(TREES[1].encode())
-    with pytest.raises(RuntimeError, match="capacity"):
-        eng.submit(tid, fn.grid_spec(2, [0, 0], [128, 128]))
+    # four times the batch capacity: four parts, one request
+    spec = fn.grid_spec(2, [0, 0], [128, 128])
+    out, mm, _ = eng.wait(eng.submit(tid, spec))
+    ref, ref_mm = alone([fn.Node.from_encoded(TREES[1].encode())], 0, spec)
+    assert_bitexact(out, ref, "oversized request")
+    assert mm == ref_mm
     with pytest.raises(RuntimeError, match="unknown tree"):
         eng.submit(tid + 5, fn.grid_spec(2, [0, 0], [8, 8]))
     with pytest.raises(ValueError):
@@ -256,4 +260,29 @@ def test_mixed_trees_alone_in_mix_and_oracle(gpu, mode):
             oref, oref_mm = trees[t].grid(list(spec.offset)[:d], list(spec.count)[:d], list(spec.step)[:d], seed=spec.seed)
             assert_bitexact(out, oref, "mixed tree %d vs oracle" % t)
             assert mm == oref_mm
+    eng.close()
+
+
+@pytest.mark.parametrize("mode", [SERVE_BATCHED, SERVE_GRAPH, SERVE_PLAIN, SERVE_PERSISTENT])
+def test_requests_beyond_the_batch_capacity_are_split(gpu, mode):
+    # a slot holds 2^18 points; larger requests run as parts over points
+    # base + [0, part) of the whole grid or image, in successive batches with
+    # other requests between them, and each equals its single call bit for
+    # bit, range included
+    nodes = [fn.Node.from_encoded(t.encode()) for t in TREES]
+    eng = ServeEngine(hbm_budget_bytes=1 << 30, max_batch_points=1 << 18, mode=mode)
+    tids = [eng.add_tree(t.encode()) for t in TREES]
+    reqs = [(0, fn.grid_spec(3, [-17.5, 40.25, 3.0], [80, 81, 79], seed=5)),           # 511,920 points: 2 parts
+            (1, fn.grid_spec(2, [100.0, -2000.0], [700, 701], seed=-3)),                # 490,700: 2 parts
+            (2, fn.grid_spec(3, [1.0, 2.0, 3.0], [16, 16, 16], seed=9)),                # small, in between
+            (3, fn.tileable_spec(1024, 600, seed=77)),                                  # 614,400: 3 parts
+            (4, fn.grid_spec(4, [1.5, -3.0, 7.0, 2.5], [24, 23, 22, 21], seed=1)),     # 255,024: 1 part
+            (1, fn.grid_spec(2, [0.5, 0.25], [1 << 18, 3], seed=11))]                   # exactly 3 parts of 2^18
+    ids = [eng.submit(tids[t], spec) for t, spec in reqs]
+    for (t, spec), rid in zip(reqs, ids):
+        out, mm, _ = eng.wait(rid)
+        ref, ref_mm = alone(nodes, t, spec)
+        assert_bitexact(out, ref, "split request tree %d, %d points" % (t, out.size))
+        assert mm == ref_mm
+    assert eng.stats()["completed"] == len(reqs)
     eng.close()
