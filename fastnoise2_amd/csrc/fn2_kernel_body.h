@@ -17,8 +17,9 @@ namespace fn2k
     using namespace fn2d;
     using namespace fn2i;
 
-    // Output stores: each value is written once and never read back by the
-    // kernel, so they go out as streaming (non-temporal) stores
+    // Output stores of whole wave rows (64 consecutive floats): each value
+    // is written once and never read back by the kernel, so they go out as
+    // streaming (non-temporal) stores
     __device__ __forceinline__ void storeOut( float* p, float v ) { __builtin_nontemporal_store( v, p ); }
 
     // BLOCK: the most threads a block of the kernel has (its launch bound);
@@ -123,7 +124,11 @@ namespace fn2k
                 fresh.R[1] = fmadd( (float)(int32_t)y, a.step[1], a.offset[1] );
                 if constexpr( D == 3 ) fresh.R[2] = fmadd( (float)(int32_t)z, a.step[2], a.offset[2] );
                 const float v = prog( a, fresh.R );
-                storeOut( a.out + ( z * ny + y ) * nx + x, v );
+                // a plain store: a wave writes 16-byte pieces of 16 lines
+                // here, which the L2 merges with its neighbours' pieces;
+                // streaming stores went out piece by piece (2.8 times the
+                // output's bytes on the warped cellular 256^3 grid)
+                a.out[( z * ny + y ) * nx + x] = v;
                 if( v == v )
                 {
                     mn = fminf( mn, v );

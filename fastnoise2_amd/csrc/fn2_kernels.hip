@@ -155,8 +155,9 @@ namespace
     // blocks per CU known: 4 rounds of what the GPU holds at once, and at
     // least 2 points per thread (measured on MI355X with the range record's
     // spread slots: 4 rounds 2-5 % faster than 2 on the cellular and serving
-    // trees, the same on FBm; 1 or 2 points per thread the same). Without
-    // it, 16 blocks per CU.
+    // trees, the same on FBm; 1 or 2 points per thread the same on large
+    // calls, and 1 slower on the 262 k-point Perlin grid: 5.0 against
+    // 4.5 us). Without it, 16 blocks per CU.
     unsigned GridFor( int64_t total, int blocksPerCU = 0, int rounds = 4 )
     {
         if( blocksPerCU <= 0 )

@@ -408,14 +408,16 @@ namespace
             {
                 // the first timing ran the interpreter; once the tree's
                 // specialised kernel is loaded, time that once more
+                // (looked up outside the engine lock: in sync mode the lookup
+                // compiles, which must not hold up the dispatcher)
+                const fn2::JitKernel* loaded = fn2::GetJitKernel( prog );
                 bool retune = false;
                 {
                     std::lock_guard<std::mutex> lk( mu );
-                    retune = !trees[treeId]->tunedSpecialised[d] && !trees[treeId]->tuning[d];
-                    if( retune && !fn2::GetJitKernel( prog ) ) retune = false;
+                    retune = loaded && !trees[treeId]->tunedSpecialised[d] && !trees[treeId]->tuning[d];
                     if( retune ) trees[treeId]->tuning[d] = true;
                 }
-                const fn2::JitKernel* k = retune ? fn2::GetJitKernel( prog ) : nullptr;
+                const fn2::JitKernel* k = retune ? loaded : nullptr;
                 if( k )
                 {
                     // the tile over threads per block and points per lane,
