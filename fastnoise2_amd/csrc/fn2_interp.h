@@ -529,7 +529,9 @@ namespace fn2i
 
     // Starting position of point `idx` into R[0..dims): grid, position arrays
     // or the tileable torus (Generator.inl:79-433).
-    template<class RF>
+    // BASE: the job may be a part of a split request (a.base != 0); the
+    // single-call kernels pass false (their base is always 0) and save the add
+    template<bool BASE = true, class RF>
     FN2D void startPosition( const fn2::GenArgs& a, uint32_t idx, RF R )
     {
         if( a.kind == fn2::SRC_POSITIONS )
@@ -538,7 +540,7 @@ namespace fn2i
         }
         else if( a.kind == fn2::SRC_TILEABLE )
         {
-            idx += a.base;
+            if( BASE ) idx += a.base;
             uint32_t yi = fastDiv( a.div[0], idx );
             uint32_t xi = idx - yi * a.div[0].d;
             // per-column and per-row torus coordinates, tabulated by the host
@@ -549,7 +551,7 @@ namespace fn2i
         }
         else
         {
-            uint32_t rem = idx + a.base;
+            uint32_t rem = BASE ? idx + a.base : idx;
             for( int ax = 0; ax < a.dims - 1; ax++ )
             {
                 uint32_t q = fastDiv( a.div[ax], rem );
@@ -564,7 +566,7 @@ namespace fn2i
     // kernels): every source writes the same D registers, from values computed
     // per source, so the register file never needs a run-time index. A
     // tileable call always runs a 4D program.
-    template<int D, class RF>
+    template<int D, bool BASE = true, class RF>
     FN2D void startPositionD( const fn2::GenArgs& a, uint32_t idx, RF R )
     {
         float p[D];
@@ -575,7 +577,7 @@ namespace fn2i
         }
         else if( D == 4 && a.kind == fn2::SRC_TILEABLE )
         {
-            idx += a.base;
+            if( BASE ) idx += a.base;
             uint32_t yi = fastDiv( a.div[0], idx );
             uint32_t xi = idx - yi * a.div[0].d;
             p[0] = a.pos[0][xi];
@@ -585,7 +587,7 @@ namespace fn2i
         }
         else
         {
-            uint32_t rem = idx + a.base;
+            uint32_t rem = BASE ? idx + a.base : idx;
 #pragma unroll
             for( int ax = 0; ax < D - 1; ax++ )
             {

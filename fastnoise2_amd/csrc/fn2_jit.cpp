@@ -65,8 +65,19 @@ namespace fn2
         // pool carries the same padding)
         constexpr int kJitConstWindow = 16;
 
-        // Programs at least this long get register fences between instructions
-        constexpr int kJitFenceMinInstructions = 128;
+        // Programs at least this long get register fences between instructions,
+        // one every kJitFenceEvery instructions (both overridable for
+        // experiments: FN2_JIT_FENCE_MIN, FN2_JIT_FENCE_EVERY)
+        int JitFenceMin()
+        {
+            static const int v = [] { const char* e = getenv( "FN2_JIT_FENCE_MIN" ); return e && atoi( e ) > 0 ? atoi( e ) : 128; }();
+            return v;
+        }
+        int JitFenceEvery()
+        {
+            static const int v = [] { const char* e = getenv( "FN2_JIT_FENCE_EVERY" ); return e && atoi( e ) > 0 ? atoi( e ) : 1; }();
+            return v;
+        }
 
         // The feature group an op needs compiled in (see fn2::kFeat*)
         int OpFeature( int op )
@@ -119,10 +130,12 @@ namespace fn2
             // per SIMD, 2.46 ms on 128^3) and 92 with (5 waves, 1.63 ms); a
             // 145-instruction tree runs 10 % faster with them, while some
             // 40-instruction trees lose 8 % to them
-            const bool fence = p.ins.size() >= (size_t)kJitFenceMinInstructions;
+            const bool fence = p.ins.size() >= (size_t)JitFenceMin();
+            const int every = JitFenceEvery();
             for( const Ins& in : p.ins )
             {
-                if( fence && &in != &p.ins[0] ) s << "fn2k::registerFence<" << p.regs << ">( R );\n";
+                const size_t i = (size_t)( &in - &p.ins[0] );
+                if( fence && i > 0 && i % (size_t)every == 0 ) s << "fn2k::registerFence<" << p.regs << ">( R );\n";
                 s << "{ constexpr fn2::Ins in = { " << (int)in.op << ", " << (int)in.dim << ", { ";
                 for( int i = 0; i < 4; i++ ) s << (int)in.p[i] << ( i < 3 ? ", " : " }, { " );
                 for( int i = 0; i < 6; i++ ) s << (int)in.r[i] << ( i < 5 ? ", " : " }, " );

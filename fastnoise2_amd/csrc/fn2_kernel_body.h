@@ -169,7 +169,7 @@ namespace fn2k
                 for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
                 {
                     RegFile fresh;
-                    startPositionD<Prog::kDims>( a, i, fresh.R );
+                    startPositionD<Prog::kDims, false>( a, i, fresh.R );
                     emit( i, prog( a, fresh.R ) );
                 }
             }
@@ -179,7 +179,7 @@ namespace fn2k
             const auto R = ldsRegs<BLOCK>( a.regs );
             for( uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < total; i += stride )
             {
-                startPosition( a, i, R );
+                startPosition<false>( a, i, R );
                 emit( i, prog( a, R ) );
             }
         }
