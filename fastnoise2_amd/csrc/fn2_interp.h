@@ -13,10 +13,12 @@
 #include "fn2_device_cellwarp.h"
 #include "fn2_kernel_args.h"
 
-// The interpreter takes the full cell walk; a host build may select the
-// pruned one to check it point by point
+// The interpreter takes the pruned cell walk, as the specialised kernels do
+// (on MI355X the warped cellular tree's interpreter kernel went from 0.633 to
+// 0.461 ms on a 256^3 grid and from 0.672 to 0.563 ms on 16.8 M positions,
+// with the same bits); defined false, it takes the full walk
 #ifndef FN2_PRUNED_CELL_WALK
-#define FN2_PRUNED_CELL_WALK false
+#define FN2_PRUNED_CELL_WALK true
 #endif
 
 // Unroll factor of a fused fractal's octave loop; 0 leaves it to the compiler
