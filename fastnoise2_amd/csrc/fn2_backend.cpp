@@ -238,18 +238,30 @@ namespace fn2
         return true;
     }
 
+    static bool SameProgram( const DeviceProgram* d, const Program& p )
+    {
+        return d->resultReg == p.resultReg && d->dims == p.dims && d->ins.size() == p.ins.size() && d->konst.size() == p.konst.size() &&
+               memcmp( d->ins.data(), p.ins.data(), p.ins.size() * sizeof( Ins ) ) == 0 &&
+               memcmp( d->konst.data(), p.konst.data(), p.konst.size() * sizeof( float ) ) == 0;
+    }
+
+    const DeviceProgram* FindDeviceProgram( const Program& p )
+    {
+        std::lock_guard<std::mutex> lk( gMu );
+        auto it = gPrograms.find( p.hash );
+        if( it == gPrograms.end() ) return nullptr;
+        for( DeviceProgram* d : it->second )
+            if( SameProgram( d, p ) ) return d;
+        return nullptr;
+    }
+
     const DeviceProgram* GetDeviceProgram( const Program& p )
     {
         std::lock_guard<std::mutex> lk( gMu );
         auto& bucket = gPrograms[p.hash];
         for( DeviceProgram* d : bucket )
         {
-            if( d->resultReg == p.resultReg && d->dims == p.dims && d->ins.size() == p.ins.size() && d->konst.size() == p.konst.size() &&
-                memcmp( d->ins.data(), p.ins.data(), p.ins.size() * sizeof( Ins ) ) == 0 &&
-                memcmp( d->konst.data(), p.konst.data(), p.konst.size() * sizeof( float ) ) == 0 )
-            {
-                return d;
-            }
+            if( SameProgram( d, p ) ) return d;
         }
         if( gProgramCount >= kMaxCachedPrograms )
         {

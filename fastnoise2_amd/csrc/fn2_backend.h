@@ -96,6 +96,8 @@ namespace fn2
     // windows; false with `err` set when it is malformed
     bool ValidateProgram( const Program& p, int& regs, std::string& err );
     const DeviceProgram* GetDeviceProgram( const Program& p );
+    // The cached device program of `p` if one exists, else null (never creates one)
+    const DeviceProgram* FindDeviceProgram( const Program& p );
     bool CompileForDevice( const FastNoise::Generator* root, int dims, const DeviceProgram*& dp, Program* hostCopy );
 
     // One generator launch: the tree's specialised kernel when it is loaded,

@@ -316,6 +316,14 @@ namespace fn2
                 return it->second.state == kReady ? &it->second.kernel : nullptr;
             }
 
+            const JitKernel* Loaded( const DeviceProgram* p, int variant )
+            {
+                if( mode == FN2_JIT_OFF || !p ) return nullptr;
+                std::lock_guard<std::mutex> lk( mu );
+                auto it = entries.find( Key{ p, variant } );
+                return it != entries.end() && it->second.state == kReady ? &it->second.kernel : nullptr;
+            }
+
             int State( const DeviceProgram* p, int variant, std::string* err )
             {
                 std::lock_guard<std::mutex> lk( mu );
@@ -400,6 +408,11 @@ namespace fn2
     const JitKernel* GetJitKernel( const DeviceProgram* p, int variant )
     {
         return Manager().Get( p, variant );
+    }
+
+    const JitKernel* LoadedJitKernel( const DeviceProgram* p, int variant )
+    {
+        return Manager().Loaded( p, variant );
     }
 
     int JitVariantState( const DeviceProgram* p, int variant )

@@ -31,6 +31,10 @@ namespace fn2
     // loaded, else null (and, in async mode, it is queued for compilation).
     const JitKernel* GetJitKernel( const DeviceProgram* p, int variant = 0 );
 
+    // The variant's kernel if it is already loaded, else null; never queues a
+    // compilation (for callers that must not allocate)
+    const JitKernel* LoadedJitKernel( const DeviceProgram* p, int variant );
+
     // Compilation state of a variant: -1 never requested, 0 queued or
     // compiling, 1 loaded, 2 failed, 3 too long to specialise.
     int JitVariantState( const DeviceProgram* p, int variant );

@@ -8,9 +8,11 @@
 //
 //   [ range record | program instructions | constant pool | tileable table ]
 //
-// The program always runs on the interpreter kernel: a tree's specialised
-// kernel would need a code-object load, which allocates. Results are the
-// same bits either way.
+// A tree whose specialised kernel earlier calls have loaded runs on that
+// kernel: its instructions and constants are literals, so it reads no
+// program from the array. Otherwise the program runs on the interpreter
+// kernel; this call never compiles or loads a specialised kernel (a
+// code-object load allocates). Results are the same bits either way.
 //
 // Each call overwrites the array of the GPU it runs on. Calls that overlap on
 // one GPU (two host threads, or two streams whose work is not ordered) race on
@@ -24,9 +26,11 @@
 
 #include "fn2_backend.h"
 #include "fn2_gpu.h"
+#include "fn2_jit.h"
 #include <FastNoise/FastNoise.h>
 
 extern "C" int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
+extern "C" unsigned fn2_grid_for( int64_t total, int blocksPerCU );
 extern "C" int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream );
 
 namespace
@@ -140,6 +144,19 @@ FASTNOISE_API int fn2GenerateDeviceNoAlloc( const void* node, const fn2GenSpec* 
     // the caller's stream, else this thread's default stream (never the
     // legacy null stream, which would order against every blocking stream)
     hipStream_t st = stream ? (hipStream_t)stream : hipStreamPerThread;
+
+    // a specialised kernel of this tree that is already loaded (looked up
+    // only: nothing is created, compiled or queued here)
+    const JitKernel* jit = nullptr;
+    if( const DeviceProgram* dp = FindDeviceProgram( p ) )
+    {
+        const int chosen = dp->variant.load( std::memory_order_relaxed );
+        jit = LoadedJitKernel( dp, 0 );
+        if( chosen )
+            if( const JitKernel* v = LoadedJitKernel( dp, chosen ) ) jit = v;
+        if( a.kind == SRC_POSITIONS && JitSharedWalkApplies( *dp ) )
+            if( const JitKernel* v = LoadedJitKernel( dp, chosen | kJitSharedWalk ) ) jit = v;
+    }
     Ins* dIns = reinterpret_cast<Ins*>( base + kInsOff );
     float* dKonst = reinterpret_cast<float*>( base + kKonstOff );
     int32_t* dKeys = reinterpret_cast<int32_t*>( base + kKeysOff );
@@ -147,9 +164,12 @@ FASTNOISE_API int fn2GenerateDeviceNoAlloc( const void* node, const fn2GenSpec* 
     // Program and tables are copied from host memory on the call's stream, so
     // the kernel below sees them; the constant pool is padded by 16 zeros,
     // as in the device program cache, for windows that start at its end.
-    if( !p.ins.empty() && !HipOk( hipMemcpyAsync( dIns, p.ins.data(), p.ins.size() * sizeof( Ins ), hipMemcpyHostToDevice, st ), "copy program" ) ) return 1;
-    p.konst.resize( p.konst.size() + 16, 0.0f );
-    if( !HipOk( hipMemcpyAsync( dKonst, p.konst.data(), p.konst.size() * sizeof( float ), hipMemcpyHostToDevice, st ), "copy constants" ) ) return 1;
+    if( !jit )
+    {
+        if( !p.ins.empty() && !HipOk( hipMemcpyAsync( dIns, p.ins.data(), p.ins.size() * sizeof( Ins ), hipMemcpyHostToDevice, st ), "copy program" ) ) return 1;
+        p.konst.resize( p.konst.size() + 16, 0.0f );
+        if( !HipOk( hipMemcpyAsync( dKonst, p.konst.data(), p.konst.size() * sizeof( float ), hipMemcpyHostToDevice, st ), "copy constants" ) ) return 1;
+    }
     if( spec->kind == FN2_SOURCE_TILEABLE )
     {
         std::vector<float> t( 2 * ( (size_t)spec->count[0] + spec->count[1] ) );
@@ -167,7 +187,15 @@ FASTNOISE_API int fn2GenerateDeviceNoAlloc( const void* node, const fn2GenSpec* 
     // the range record starts empty: a fill kernel, not a memset, so that the
     // call can be recorded into a graph
     if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( dKeys, 1, kRangeStrideSpread, st ), "launch min/max init" ) ) return 1;
-    if( !HipOk( (hipError_t)fn2_launch_generate( &a, p.dimMask, p.featMask, st ), "launch generator" ) ) return 1;
+    if( jit )
+    {
+        GenArgs copy = a;
+        void* params[] = { &copy };
+        if( !HipOk( hipModuleLaunchKernel( jit->single, fn2_grid_for( total, jit->blocksPerCU ), 1, 1, 256, 1, 1, 0, st, params, nullptr ),
+                    "launch specialised generator" ) )
+            return 1;
+    }
+    else if( !HipOk( (hipError_t)fn2_launch_generate( &a, p.dimMask, p.featMask, st ), "launch generator" ) ) return 1;
     if( outputMinMax )
     {
         int32_t keys[kCallRangeInts];

@@ -52,7 +52,9 @@ FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, f
  * The call allocates no device memory: the compiled program, the range
  * record and GenTileable2D's torus table go into one device array of
  * fn2NoAllocScratchBytes() bytes that each GPU holds until the process ends.
- * It runs the interpreter kernel (the same bits as the specialised kernel).
+ * It runs the tree's specialised kernel when earlier calls have loaded it
+ * (it never compiles or loads one itself), else the interpreter kernel: the
+ * same bits either way.
  * Limits: at most FN2_NOALLOC_MAX_CONSTANTS constants in the compiled tree
  * and xSize + ySize <= FN2_NOALLOC_MAX_TILE_AXES for tileable specs; larger
  * requests fail with an error and launch nothing. Positions must be device

@@ -78,21 +78,33 @@ def test_no_alloc_matches_generate_device(gpu):
     del keep
 
 
-def test_no_alloc_specialised_kernel_same_bits(gpu):
-    # fn2GenerateDevice on the tree's specialised kernel, the no-allocation
-    # call on the interpreter: the same bits
-    tree = W.warped_cellular_3d()
+def test_no_alloc_before_and_after_the_specialised_kernel_loads(gpu):
+    # before the tree's specialised kernel is loaded the no-allocation call
+    # runs the interpreter, and queues no compilation; once an ordinary call
+    # has loaded the kernel, it runs that kernel: the same bits every time.
+    # Specialised kernels are cached per program for the process, so the tree
+    # is one no other test builds (warp amplitude 23.5)
+    cell = S("CellularDistance", feature_scale=50, distance_function="Euclidean",
+             return_type="Index0Add1", distance_index_0=0, distance_index_1=1, grid_jitter=1.0)
+    tree = S("DomainWarpGradient", source=cell, feature_scale=100, warp_amplitude=23.5)
     node = fn.Node.from_encoded(tree.encode())
     spec = fn.grid_spec(3, [3.0, -7.0, 1.0], [64, 48, 32], seed=1337)
-    old = fn.set_jit_mode(fn.JIT_SYNC)
+    n = 64 * 48 * 32
+    old = fn.set_jit_mode(fn.JIT_OFF)
     try:
-        ref, ref_mm = _call("fn2GenerateDevice", node, spec, 64 * 48 * 32)
+        assert node.jit_state(3) == -1, "another test loaded this tree's kernel"
+        ref, ref_mm = _call("fn2GenerateDevice", node, spec, n)
+        fn.set_jit_mode(fn.JIT_SYNC)
+        interp, interp_mm = _call("fn2GenerateDeviceNoAlloc", node, spec, n)
+        assert node.jit_state(3) == -1, "the no-allocation call requested a compilation"
+        _call("fn2GenerateDevice", node, spec, n)
         assert node.jit_state(3) == 1
+        spec_out, spec_mm = _call("fn2GenerateDeviceNoAlloc", node, spec, n)
     finally:
         fn.set_jit_mode(old)
-    got, got_mm = _call("fn2GenerateDeviceNoAlloc", node, spec, 64 * 48 * 32)
-    assert_bitexact(got, ref, "specialised vs no-allocation")
-    assert got_mm == ref_mm
+    assert_bitexact(interp, ref, "no-allocation call on the interpreter")
+    assert_bitexact(spec_out, ref, "no-allocation call on the specialised kernel")
+    assert interp_mm == ref_mm and spec_mm == ref_mm
 
 
 def test_no_alloc_limits_and_empty_calls(gpu):
