@@ -14,10 +14,11 @@ namespace fn2d
     };
 
     // ---------------- Perlin (Perlin.inl:7-125) ----------------
+    template<bool G = true>
     FN2D float perlin2( int32_t seed, float x, float y )
     {
         float xs = ffloor( x ), ys = ffloor( y );
-        int32_t x0 = imul( floorToInt( xs ), kPX ), y0 = imul( floorToInt( ys ), kPY );
+        int32_t x0 = imul( floorToInt<G>( xs ), kPX ), y0 = imul( floorToInt<G>( ys ), kPY );
         int32_t x1 = iadd( x0, kPX ), y1 = iadd( y0, kPY );
         float xf0 = x - xs, yf0 = y - ys;
         float xf1 = xf0 - 1.0f, yf1 = yf0 - 1.0f;
@@ -25,10 +26,11 @@ namespace fn2d
         return lerp( lerp( gradDot2( hash2( seed, x0, y0 ), xf0, yf0 ), gradDot2( hash2( seed, x1, y0 ), xf1, yf0 ), xq ),
                      lerp( gradDot2( hash2( seed, x0, y1 ), xf0, yf1 ), gradDot2( hash2( seed, x1, y1 ), xf1, yf1 ), xq ), yq );
     }
+    template<bool G = true>
     FN2D float perlin3( int32_t seed, float x, float y, float z )
     {
         float xs = ffloor( x ), ys = ffloor( y ), zs = ffloor( z );
-        int32_t x0 = imul( floorToInt( xs ), kPX ), y0 = imul( floorToInt( ys ), kPY ), z0 = imul( floorToInt( zs ), kPZ );
+        int32_t x0 = imul( floorToInt<G>( xs ), kPX ), y0 = imul( floorToInt<G>( ys ), kPY ), z0 = imul( floorToInt<G>( zs ), kPZ );
         int32_t x1 = iadd( x0, kPX ), y1 = iadd( y0, kPY ), z1 = iadd( z0, kPZ );
         float xf0 = x - xs, yf0 = y - ys, zf0 = z - zs;
         float xf1 = xf0 - 1.0f, yf1 = yf0 - 1.0f, zf1 = zf0 - 1.0f;
@@ -39,10 +41,11 @@ namespace fn2d
                          lerp( gradDot3( hash3( seed, x0, y1, z1 ), xf0, yf1, zf1 ), gradDot3( hash3( seed, x1, y1, z1 ), xf1, yf1, zf1 ), xq ), yq );
         return lerp( l0, l1, zq );
     }
+    template<bool G = true>
     FN2D float perlin4( int32_t seed, float x, float y, float z, float w )
     {
         float xs = ffloor( x ), ys = ffloor( y ), zs = ffloor( z ), ws = ffloor( w );
-        int32_t x0 = imul( floorToInt( xs ), kPX ), y0 = imul( floorToInt( ys ), kPY ), z0 = imul( floorToInt( zs ), kPZ ), w0 = imul( floorToInt( ws ), kPW );
+        int32_t x0 = imul( floorToInt<G>( xs ), kPX ), y0 = imul( floorToInt<G>( ys ), kPY ), z0 = imul( floorToInt<G>( zs ), kPZ ), w0 = imul( floorToInt<G>( ws ), kPW );
         int32_t x1 = iadd( x0, kPX ), y1 = iadd( y0, kPY ), z1 = iadd( z0, kPZ ), w1 = iadd( w0, kPW );
         float xf0 = x - xs, yf0 = y - ys, zf0 = z - zs, wf0 = w - ws;
         float xf1 = xf0 - 1.0f, yf1 = yf0 - 1.0f, zf1 = zf0 - 1.0f, wf1 = wf0 - 1.0f;
@@ -68,19 +71,21 @@ namespace fn2d
     }
 
     // ---------------- Value (Value.inl:7-96) ----------------
+    template<bool G = true>
     FN2D float value2( int32_t seed, float x, float y )
     {
         float xs = ffloor( x ), ys = ffloor( y );
-        int32_t x0 = imul( floorToInt( xs ), kPX ), y0 = imul( floorToInt( ys ), kPY );
+        int32_t x0 = imul( floorToInt<G>( xs ), kPX ), y0 = imul( floorToInt<G>( ys ), kPY );
         int32_t x1 = iadd( x0, kPX ), y1 = iadd( y0, kPY );
         float xh = interpHermite( x - xs ), yh = interpHermite( y - ys );
         return lerp( lerp( valueCoord2( seed, x0, y0 ), valueCoord2( seed, x1, y0 ), xh ),
                      lerp( valueCoord2( seed, x0, y1 ), valueCoord2( seed, x1, y1 ), xh ), yh );
     }
+    template<bool G = true>
     FN2D float value3( int32_t seed, float x, float y, float z )
     {
         float xs = ffloor( x ), ys = ffloor( y ), zs = ffloor( z );
-        int32_t x0 = imul( floorToInt( xs ), kPX ), y0 = imul( floorToInt( ys ), kPY ), z0 = imul( floorToInt( zs ), kPZ );
+        int32_t x0 = imul( floorToInt<G>( xs ), kPX ), y0 = imul( floorToInt<G>( ys ), kPY ), z0 = imul( floorToInt<G>( zs ), kPZ );
         int32_t x1 = iadd( x0, kPX ), y1 = iadd( y0, kPY ), z1 = iadd( z0, kPZ );
         float xh = interpHermite( x - xs ), yh = interpHermite( y - ys ), zh = interpHermite( z - zs );
         float l0 = lerp( lerp( valueCoord3( seed, x0, y0, z0 ), valueCoord3( seed, x1, y0, z0 ), xh ),
@@ -89,10 +94,11 @@ namespace fn2d
                          lerp( valueCoord3( seed, x0, y1, z1 ), valueCoord3( seed, x1, y1, z1 ), xh ), yh );
         return lerp( l0, l1, zh );
     }
+    template<bool G = true>
     FN2D float value4( int32_t seed, float x, float y, float z, float w )
     {
         float xs = ffloor( x ), ys = ffloor( y ), zs = ffloor( z ), ws = ffloor( w );
-        int32_t x0 = imul( floorToInt( xs ), kPX ), y0 = imul( floorToInt( ys ), kPY ), z0 = imul( floorToInt( zs ), kPZ ), w0 = imul( floorToInt( ws ), kPW );
+        int32_t x0 = imul( floorToInt<G>( xs ), kPX ), y0 = imul( floorToInt<G>( ys ), kPY ), z0 = imul( floorToInt<G>( zs ), kPZ ), w0 = imul( floorToInt<G>( ws ), kPW );
         int32_t x1 = iadd( x0, kPX ), y1 = iadd( y0, kPY ), z1 = iadd( z0, kPZ ), w1 = iadd( w0, kPW );
         float xh = interpHermite( x - xs ), yh = interpHermite( y - ys ), zh = interpHermite( z - zs ), wh = interpHermite( w - ws );
         float v[2];
@@ -116,6 +122,7 @@ namespace fn2d
     // ---------------- Simplex (Simplex.inl:7-286) ----------------
     // Returns the un-scaled value; the three falloffs and gradients are combined
     // in the reference's order for each dimension.
+    template<bool G = true>
     FN2D float simplex2( int32_t seed, float x, float y )
     {
         constexpr double kSkew2 = 1.0 / ( kRoot3 + 1.0 );
@@ -124,7 +131,7 @@ namespace fn2d
         float xSk = x + skewDelta, ySk = y + skewDelta;
         float xB = ffloor( xSk ), yB = ffloor( ySk );
         float dxS = xSk - xB, dyS = ySk - yB;
-        int32_t xP = imul( floorToInt( xB ), kPX ), yP = imul( floorToInt( yB ), kPY );
+        int32_t xP = imul( floorToInt<G>( xB ), kPX ), yP = imul( floorToInt<G>( yB ), kPY );
         bool xGEy = dxS >= dyS;
         float unskewDelta = (float)kUnskew2 * ( dxS + dyS );
         float dx0 = dxS + unskewDelta, dy0 = dyS + unskewDelta;
@@ -142,6 +149,7 @@ namespace fn2d
         float g2 = gradDot2( hash2( seed, iadd( xP, kPX ), iadd( yP, kPY ) ), dx2, dy2 );
         return fmadd( g0, f0, fmadd( g1, f1, g2 * f2 ) );
     }
+    template<bool G = true>
     FN2D float simplex3( int32_t seed, float x, float y, float z )
     {
         constexpr double kReflectUnskew3 = -1.0 / 2.0;
@@ -149,7 +157,7 @@ namespace fn2d
         float xSk = x + skewDelta, ySk = y + skewDelta, zSk = z + skewDelta;
         float xB = ffloor( xSk ), yB = ffloor( ySk ), zB = ffloor( zSk );
         float dxS = xSk - xB, dyS = ySk - yB, dzS = zSk - zB;
-        int32_t xP = imul( floorToInt( xB ), kPX ), yP = imul( floorToInt( yB ), kPY ), zP = imul( floorToInt( zB ), kPZ );
+        int32_t xP = imul( floorToInt<G>( xB ), kPX ), yP = imul( floorToInt<G>( yB ), kPY ), zP = imul( floorToInt<G>( zB ), kPZ );
         bool xGEy = dxS >= dyS, yGEz = dyS >= dzS, xGEz = dxS >= dzS;
         float unskewDelta = (float)kReflectUnskew3 * ( dxS + dyS + dzS );
         float dx0 = dxS + unskewDelta, dy0 = dyS + unskewDelta, dz0 = dzS + unskewDelta;
@@ -177,6 +185,7 @@ namespace fn2d
         float g3 = gradDot3( hash3( seed, iadd( xP, kPX ), iadd( yP, kPY ), iadd( zP, kPZ ) ), dx3, dy3, dz3 );
         return fmadd( g3, f3, fmadd( g2, f2, fmadd( g1, f1, g0 * f0 ) ) );
     }
+    template<bool G = true>
     FN2D float simplex4( int32_t seed, float x, float y, float z, float w )
     {
         constexpr double kSkew4 = 1.0 / ( kRoot5 + 1.0 );
@@ -185,7 +194,7 @@ namespace fn2d
         float xSk = x + skewDelta, ySk = y + skewDelta, zSk = z + skewDelta, wSk = w + skewDelta;
         float xB = ffloor( xSk ), yB = ffloor( ySk ), zB = ffloor( zSk ), wB = ffloor( wSk );
         float dxS = xSk - xB, dyS = ySk - yB, dzS = zSk - zB, dwS = wSk - wB;
-        int32_t xP = imul( floorToInt( xB ), kPX ), yP = imul( floorToInt( yB ), kPY ), zP = imul( floorToInt( zB ), kPZ ), wP = imul( floorToInt( wB ), kPW );
+        int32_t xP = imul( floorToInt<G>( xB ), kPX ), yP = imul( floorToInt<G>( yB ), kPY ), zP = imul( floorToInt<G>( zB ), kPZ ), wP = imul( floorToInt<G>( wB ), kPW );
         float unskewDelta = (float)kUnskew4 * ( dxS + dyS + dzS + dwS );
         float d0[4] = { dxS + unskewDelta, dyS + unskewDelta, dzS + unskewDelta, dwS + unskewDelta };
         // rank of each axis among the four (ties resolved by axis order)
@@ -236,7 +245,7 @@ namespace fn2d
 
     // ---------------- SuperSimplex walks (Simplex.inl:291-936) ----------------
     // vertex( hash, d[], falloff^4, first )
-    template<typename VTX>
+    template<bool G = true, typename VTX>
     FN2D void superWalk2( int32_t seed, float x, float y, VTX&& vertex )
     {
         constexpr double kSkew2 = 1.0 / ( kRoot3 + 1.0 );
@@ -246,7 +255,7 @@ namespace fn2d
         float xSk = x + skewDelta, ySk = y + skewDelta;
         float xB = ffloor( xSk ), yB = ffloor( ySk );
         float dxS = xSk - xB, dyS = ySk - yB;
-        int32_t xP = imul( floorToInt( xB ), kPX ), yP = imul( floorToInt( yB ), kPY );
+        int32_t xP = imul( floorToInt<G>( xB ), kPX ), yP = imul( floorToInt<G>( yB ), kPY );
         bool fXY = dxS + dyS > 1.0f;
         float boundary = fXY ? -1.0f : 0.0f;
         bool fX = fmadd( dxS, -2.0f, dyS ) < boundary;
@@ -296,7 +305,7 @@ namespace fn2d
 
     FN2D int32_t signBitIf( bool m ) { return m ? (int32_t)0x80000000u : 0; }
 
-    template<typename VTX>
+    template<bool G = true, typename VTX>
     FN2D void superWalk3( int32_t seed, float x, float y, float z, VTX&& vertex )
     {
         constexpr double kReflectUnskew3 = -1.0 / 2.0;
@@ -320,7 +329,7 @@ namespace fn2d
             if( bits & 2 ) { yB += 1.0f; dyS -= 1.0f; }
             if( bits & 4 ) { zB += 1.0f; dzS -= 1.0f; }
         }
-        int32_t xP = imul( floorToInt( xB ), kPX ), yP = imul( floorToInt( yB ), kPY ), zP = imul( floorToInt( zB ), kPZ );
+        int32_t xP = imul( floorToInt<G>( xB ), kPX ), yP = imul( floorToInt<G>( yB ), kPY ), zP = imul( floorToInt<G>( zB ), kPZ );
         float sum = dxS + dyS + dzS;
         float tud = (float)kTwiceUnskew3 * sum;
         float xN = dxS + tud, yN = dyS + tud, zN = dzS + tud, xyzN = -tud;
@@ -393,7 +402,7 @@ namespace fn2d
         }
     }
 
-    template<typename VTX>
+    template<bool G = true, typename VTX>
     FN2D void superWalk4( int32_t seed, float x, float y, float z, float w, VTX&& vertex )
     {
         constexpr double kSkew4 = 1.0 / ( kRoot5 + 1.0 );
@@ -425,7 +434,7 @@ namespace fn2d
             if( bits & 4 ) { zB += 1.0f; dzS -= 1.0f; }
             if( bits & 8 ) { wB += 1.0f; dwS -= 1.0f; }
         }
-        int32_t P[4] = { imul( floorToInt( xB ), kPX ), imul( floorToInt( yB ), kPY ), imul( floorToInt( zB ), kPZ ), imul( floorToInt( wB ), kPW ) };
+        int32_t P[4] = { imul( floorToInt<G>( xB ), kPX ), imul( floorToInt<G>( yB ), kPY ), imul( floorToInt<G>( zB ), kPZ ), imul( floorToInt<G>( wB ), kPW ) };
         const int32_t prime[4] = { kPX, kPY, kPZ, kPW };
         float sum = dxS + dyS + dzS + dwS;
         float tud = (float)kTwiceUnskew4 * sum;
@@ -494,28 +503,31 @@ namespace fn2d
         single( 3 );
     }
 
+    template<bool G = true>
     FN2D float superSimplex2( int32_t seed, float x, float y )
     {
         float value = 0.0f;
-        superWalk2( seed, x, y, [&]( int32_t h, const float* d, float f, bool first ) {
+        superWalk2<G>( seed, x, y, [&]( int32_t h, const float* d, float f, bool first ) {
             float g = gradDot2( h, d[0], d[1] );
             value = first ? f * g : fmadd( f, g, value );
         } );
         return value;
     }
+    template<bool G = true>
     FN2D float superSimplex3( int32_t seed, float x, float y, float z )
     {
         float value = 0.0f;
-        superWalk3( seed, x, y, z, [&]( int32_t h, const float* d, float f, bool first ) {
+        superWalk3<G>( seed, x, y, z, [&]( int32_t h, const float* d, float f, bool first ) {
             float g = gradDot3( h, d[0], d[1], d[2] );
             value = first ? f * g : fmadd( f, g, value );
         } );
         return value;
     }
+    template<bool G = true>
     FN2D float superSimplex4( int32_t seed, float x, float y, float z, float w )
     {
         float value = 0.0f;
-        superWalk4( seed, x, y, z, w, [&]( int32_t h, const float* d, float f, bool first ) {
+        superWalk4<G>( seed, x, y, z, w, [&]( int32_t h, const float* d, float f, bool first ) {
             float g = gradDot4( h, d[0], d[1], d[2], d[3] );
             value = first ? f * g : fmadd( f, g, value );
         } );

@@ -60,18 +60,22 @@ namespace fn2d
     // instead (NaN -> 0, >= 2^31 -> INT_MAX, < -2^31 -> INT_MIN), so only NaN
     // and a >= 2^31 need the select. The hardware instruction is issued
     // directly because an out-of-range C++ float->int conversion is undefined.
+    // GUARD = false: the caller has shown |a| < 2^31, so the select is left out
+    template<bool GUARD = true>
     FN2D int32_t cvtIntegral( float a )
     {
 #if defined( __HIP_DEVICE_COMPILE__ )
         int32_t v;
         asm volatile( "v_cvt_i32_f32 %0, %1" : "=v"( v ) : "v"( a ) );
+        if constexpr( !GUARD ) return v;
 #else
         int32_t v = a >= -2147483648.0f && a < 2147483648.0f ? (int32_t)a : (int32_t)0x80000000u;
 #endif
         return a < 2147483648.0f ? v : (int32_t)0x80000000u;
     }
     // floor() results are integral, so the truncating convert is exact
-    FN2D int32_t floorToInt( float a ) { return cvtIntegral( a ); }
+    template<bool GUARD = true>
+    FN2D int32_t floorToInt( float a ) { return cvtIntegral<GUARD>( a ); }
     // FS::Convert<int32_t> of a non-integer: round to nearest even
     FN2D int32_t roundToInt( float a ) { return cvtIntegral( __builtin_rintf( a ) ); }
     FN2D float invSqrt( float a ) { return 1.0f / __builtin_sqrtf( a ); }
@@ -138,22 +142,37 @@ namespace fn2d
     template<int D> FN2D float invSqrtCell( float x ) { return recipPosRN( D == 2 ? sqrtPosRN( x ) : sqrtPosRNFromBelow( x ) ); }
     FN2D float recip( float a ) { return 1.0f / a; }
 
+    // a ^ b ^ c as one v_bitop3_b32: the compiler splits three-input xors
+    // into two v_xor_b32 on gfx950 (it has no v_xor3_b32)
+    FN2D int32_t ixor3( int32_t a, int32_t b, int32_t c )
+    {
+#if defined( __HIP_DEVICE_COMPILE__ )
+        return (int32_t)__builtin_amdgcn_bitop3_b32( (uint32_t)a, (uint32_t)b, (uint32_t)c, 0x96 );
+#else
+        return a ^ b ^ c;
+#endif
+    }
+    // the seed and the primed coordinates xored together (Utils.inl:221-250)
+    FN2D int32_t hashIn( int32_t seed, int32_t x, int32_t y ) { return ixor3( seed, x, y ); }
+    FN2D int32_t hashIn( int32_t seed, int32_t x, int32_t y, int32_t z ) { return ixor3( seed, x, y ^ z ); }
+    FN2D int32_t hashIn( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return ixor3( ixor3( seed, x, y ), z, w ); }
+
     // ---- hashing (Utils.inl:221-250)
     FN2D int32_t hashFinal( int32_t h )
     {
         h = imul( h, kHashMul );
         return ( h >> 15 ) ^ h;   // arithmetic shift
     }
-    FN2D int32_t hash2( int32_t seed, int32_t x, int32_t y ) { return hashFinal( seed ^ ( x ^ y ) ); }
-    FN2D int32_t hash3( int32_t seed, int32_t x, int32_t y, int32_t z ) { return hashFinal( seed ^ ( x ^ y ^ z ) ); }
-    FN2D int32_t hash4( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return hashFinal( seed ^ ( x ^ y ^ z ^ w ) ); }
-    FN2D int32_t hashHB2( int32_t seed, int32_t x, int32_t y ) { return imul( seed ^ ( x ^ y ), kHashMul ); }
-    FN2D int32_t hashHB3( int32_t seed, int32_t x, int32_t y, int32_t z ) { return imul( seed ^ ( x ^ y ^ z ), kHashMul ); }
-    FN2D int32_t hashHB4( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return imul( seed ^ ( x ^ y ^ z ^ w ), kHashMul ); }
+    FN2D int32_t hash2( int32_t seed, int32_t x, int32_t y ) { return hashFinal( hashIn( seed, x, y ) ); }
+    FN2D int32_t hash3( int32_t seed, int32_t x, int32_t y, int32_t z ) { return hashFinal( hashIn( seed, x, y, z ) ); }
+    FN2D int32_t hash4( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return hashFinal( hashIn( seed, x, y, z, w ) ); }
+    FN2D int32_t hashHB2( int32_t seed, int32_t x, int32_t y ) { return imul( hashIn( seed, x, y ), kHashMul ); }
+    FN2D int32_t hashHB3( int32_t seed, int32_t x, int32_t y, int32_t z ) { return imul( hashIn( seed, x, y, z ), kHashMul ); }
+    FN2D int32_t hashHB4( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return imul( hashIn( seed, x, y, z, w ), kHashMul ); }
     FN2D float valueCoordFinal( int32_t h ) { return (float)imul( h, imul( h, kHashMul ) ); }
-    FN2D float valueCoord2( int32_t seed, int32_t x, int32_t y ) { return valueCoordFinal( seed ^ ( x ^ y ) ); }
-    FN2D float valueCoord3( int32_t seed, int32_t x, int32_t y, int32_t z ) { return valueCoordFinal( seed ^ ( x ^ y ^ z ) ); }
-    FN2D float valueCoord4( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return valueCoordFinal( seed ^ ( x ^ y ^ z ^ w ) ); }
+    FN2D float valueCoord2( int32_t seed, int32_t x, int32_t y ) { return valueCoordFinal( hashIn( seed, x, y ) ); }
+    FN2D float valueCoord3( int32_t seed, int32_t x, int32_t y, int32_t z ) { return valueCoordFinal( hashIn( seed, x, y, z ) ); }
+    FN2D float valueCoord4( int32_t seed, int32_t x, int32_t y, int32_t z, int32_t w ) { return valueCoordFinal( hashIn( seed, x, y, z, w ) ); }
 
     // ---- interpolants (Utils.inl:252-265)
     FN2D float lerp( float a, float b, float t ) { return fmadd( t, b - a, a ); }

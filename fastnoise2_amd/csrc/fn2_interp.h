@@ -59,36 +59,38 @@ namespace fn2i
     template<int OP, int... Ops>
     constexpr bool opIn() { return OP < 0 || ( ( OP == Ops ) || ... ); }
 
-    // Lattice noise on explicit coordinates; LEAF as OP above
-    template<int D, int LEAF = -1>
+    // Lattice noise on explicit coordinates; LEAF as OP above. G = false
+    // leaves out the out-of-range select of the float -> int conversions, for
+    // coordinates shown to lie below kUnguardedCoord in magnitude.
+    template<int D, int LEAF = -1, bool G = true>
     FN2D float leafRaw( int op, int32_t seed, const float* x )
     {
         if constexpr( opIn<LEAF, fn2::OP_PERLIN>() )
             if( LEAF >= 0 || op == fn2::OP_PERLIN )
             {
-                if constexpr( D == 2 ) return perlin2( seed, x[0], x[1] );
-                else if constexpr( D == 3 ) return perlin3( seed, x[0], x[1], x[2] );
-                else return perlin4( seed, x[0], x[1], x[2], x[3] );
+                if constexpr( D == 2 ) return perlin2<G>( seed, x[0], x[1] );
+                else if constexpr( D == 3 ) return perlin3<G>( seed, x[0], x[1], x[2] );
+                else return perlin4<G>( seed, x[0], x[1], x[2], x[3] );
             }
         if constexpr( opIn<LEAF, fn2::OP_VALUE>() )
             if( LEAF >= 0 || op == fn2::OP_VALUE )
             {
-                if constexpr( D == 2 ) return value2( seed, x[0], x[1] );
-                else if constexpr( D == 3 ) return value3( seed, x[0], x[1], x[2] );
-                else return value4( seed, x[0], x[1], x[2], x[3] );
+                if constexpr( D == 2 ) return value2<G>( seed, x[0], x[1] );
+                else if constexpr( D == 3 ) return value3<G>( seed, x[0], x[1], x[2] );
+                else return value4<G>( seed, x[0], x[1], x[2], x[3] );
             }
         if constexpr( opIn<LEAF, fn2::OP_SIMPLEX>() )
             if( LEAF >= 0 || op == fn2::OP_SIMPLEX )
             {
-                if constexpr( D == 2 ) return simplex2( seed, x[0], x[1] );
-                else if constexpr( D == 3 ) return simplex3( seed, x[0], x[1], x[2] );
-                else return simplex4( seed, x[0], x[1], x[2], x[3] );
+                if constexpr( D == 2 ) return simplex2<G>( seed, x[0], x[1] );
+                else if constexpr( D == 3 ) return simplex3<G>( seed, x[0], x[1], x[2] );
+                else return simplex4<G>( seed, x[0], x[1], x[2], x[3] );
             }
         if constexpr( opIn<LEAF, fn2::OP_SUPERSIMPLEX>() )
         {
-            if constexpr( D == 2 ) return superSimplex2( seed, x[0], x[1] );
-            else if constexpr( D == 3 ) return superSimplex3( seed, x[0], x[1], x[2] );
-            else return superSimplex4( seed, x[0], x[1], x[2], x[3] );
+            if constexpr( D == 2 ) return superSimplex2<G>( seed, x[0], x[1] );
+            else if constexpr( D == 3 ) return superSimplex3<G>( seed, x[0], x[1], x[2] );
+            else return superSimplex4<G>( seed, x[0], x[1], x[2], x[3] );
         }
         return 0.0f;
     }
@@ -108,8 +110,8 @@ namespace fn2i
     // A specialised kernel knows the octave count; FN2_OCTAVE_UNROLL (0: the
     // compiler's choice) sets how far the loop is unrolled, a per-tree tuning
     // knob of the serving engine.
-    template<int D, int LEAF = -1, class RF = const float*>
-    FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, RF R, int p )
+    template<int D, int LEAF, bool G>
+    FN2D float fractalOctaves( const Ins& in, const float* K, int32_t seed, const float* start )
     {
         const int leaf = in.p[0], octaves = in.p[1];
         const bool ridged = in.p[2] != 0;
@@ -117,12 +119,12 @@ namespace fn2i
         const float gain = K[4], ws = K[5], lac = K[6];
         float pos[D];
 #pragma unroll
-        for( int a = 0; a < D; a++ ) pos[a] = R[p + a];
+        for( int a = 0; a < D; a++ ) pos[a] = start[a];
         auto octave = [&]( int32_t s ) {
             float x[D];
 #pragma unroll
             for( int a = 0; a < D; a++ ) x[a] = pos[a] * freq;
-            return scaleOutput( leafRaw<D, LEAF>( leaf, s, x ), nMin, oScale, rMin );
+            return scaleOutput( leafRaw<D, LEAF, G>( leaf, s, x ), nMin, oScale, rMin );
         };
         float amp = 1.0f;
         float noise = octave( seed );
@@ -155,6 +157,37 @@ namespace fn2i
             else sum += noise * amp;
         }
         return sum;
+    }
+
+    // Lattice coordinates below this magnitude keep every lattice's floor
+    // argument (at most 2.24 times the largest coordinate, for the 4D skew)
+    // far below 2^31, where the float -> int conversion needs no select
+    constexpr float kUnguardedCoord = 268435456.0f;   // 2^28
+
+    template<int D, int LEAF = -1, class RF = const float*>
+    FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, RF R, int p )
+    {
+        float pos[D];
+#pragma unroll
+        for( int a = 0; a < D; a++ ) pos[a] = R[p + a];
+        if constexpr( LEAF >= 0 )
+        {
+            // A specialised kernel folds this bound to a literal: octave k
+            // reads pos * lac^k * freq (rounded at each step, far inside the
+            // margin), so |pos| < limit keeps every octave's coordinates below
+            // kUnguardedCoord. The comparisons fail for NaN and infinities,
+            // whose waves take the guarded loop, as does any wave when the
+            // bound overflows.
+            const int octaves = in.p[1];
+            float growth = fabs_( K[0] );
+            for( int i = 1; i < octaves; i++ ) growth *= fmaxf_( 1.0f, fabs_( K[6] ) );
+            const float limit = kUnguardedCoord / growth;
+            bool small = true;
+#pragma unroll
+            for( int a = 0; a < D; a++ ) small = small && fabs_( pos[a] ) < limit;
+            if( waveAll( small ) ) return fractalOctaves<D, LEAF, false>( in, K, seed, pos );
+        }
+        return fractalOctaves<D, LEAF, true>( in, K, seed, pos );
     }
 
     // SLOTS: distance slots CellularDistance keeps (4, or fewer when the

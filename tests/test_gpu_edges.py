@@ -129,6 +129,38 @@ def test_oversized_request_is_rejected(gpu):
     assert "2^31" in fn.last_error()
 
 
+@pytest.mark.parametrize("leaf", ["Simplex", "SuperSimplex", "Perlin", "Value"])
+@pytest.mark.parametrize("dims", [2, 3, 4])
+def test_fused_fractal_coordinate_bound(gpu, leaf, dims):
+    # a specialised fused fractal drops the out-of-range select of its float
+    # -> int conversions for waves whose every octave stays below 2^28 in
+    # magnitude (pos * lac^k * freq); waves with a lane beyond that take the
+    # guarded loop. Positions just inside and outside the bound, whole waves
+    # of each and single lanes over it among small ones, against the oracle.
+    freq, lac, octaves = 2.0, 2.5, 4
+    spec = S("FractalFBm", source=S(leaf, feature_scale=1.0 / freq), octaves=octaves, lacunarity=lac)
+    node = fn.Node.from_encoded(spec.encode())
+    limit = 2.0 ** 28 / (freq * lac ** (octaves - 1))
+    rng = np.random.default_rng(dims * 7 + len(leaf))
+    n = 4096
+    pos = [rng.uniform(-1000.0, 1000.0, n) for _ in range(dims)]
+    for p in pos:
+        p[1024:2048] = rng.uniform(-0.999, 0.999, 1024) * limit            # inside, near the bound
+        p[3072:4096] = rng.choice([-1.0, 1.0], 1024) * rng.uniform(1.001, 7.0, 1024) * limit
+    for k, v in enumerate([1.01 * limit, -1.5 * limit, 3.0e9, -2.0 ** 31, 0.9999 * limit, 65.0 * limit]):
+        pos[k % dims][2048 + 64 * k + (11 * k) % 64] = v                   # one lane per wave
+    pos = [p.astype(np.float32) for p in pos]
+    old = fn.set_jit_mode(fn.JIT_SYNC)
+    try:
+        out, mm = node.gen_position_array(pos, seed=-77)
+        assert node.jit_state(dims) == 1, fn.last_error()
+    finally:
+        fn.set_jit_mode(old)
+    ref, ref_mm = oracle.Tree(spec.encode()).positions(pos, seed=-77)
+    assert_bitexact(out, ref, "%s %dD fractal near the coordinate bound" % (leaf, dims))
+    assert mm == ref_mm
+
+
 @pytest.mark.parametrize("kind", ["FractalFBm", "FractalRidged"])
 @pytest.mark.parametrize("ws", [0.0, -0.0, 0.6])
 @pytest.mark.parametrize("jit", [fn.JIT_OFF, fn.JIT_SYNC], ids=["interpreter", "specialised"])
