@@ -136,15 +136,14 @@ namespace fn2i
         for( int i = 1; i < octaves; i++ )
         {
             seed = iadd( seed, 1 );
-            if( ws == 0.0f )
-            {
-                // lerp( 1, v, 0 ) = fma( 0, v - 1, 1 ) is exactly 1 for a
-                // finite v and NaN otherwise, and v (either form) is finite
-                // exactly when noise is: 2 instructions instead of 5 (a
-                // specialised kernel knows ws, so only this side is compiled)
-                if( !( fabs_( noise ) <= 3.40282347e38f ) ) amp = __builtin_nanf( "" );
-            }
-            else amp *= ridged ? lerp( 1.0f, 1.0f - noise, ws ) : lerp( 1.0f, ( noise + 1.0f ) * 0.5f, ws );
+            // lerp( 1, v, 0 ) = fma( 0, v - 1, 1 ) is exactly 1 for a finite
+            // v and NaN otherwise. A lattice leaf's octave value is finite or
+            // NaN (its outputs are bounded), and a NaN octave has already made
+            // the sum NaN for good, so with weighted strength zero the factor
+            // changes no result and is left out (a specialised kernel knows
+            // ws, so only one side is compiled)
+            if( ws != 0.0f )
+                amp *= ridged ? lerp( 1.0f, 1.0f - noise, ws ) : lerp( 1.0f, ( noise + 1.0f ) * 0.5f, ws );
             amp *= gain;
 #pragma unroll
             for( int a = 0; a < D; a++ ) pos[a] = pos[a] * lac;
