@@ -28,12 +28,7 @@
 #include "fn2_jit.h"
 #include "fn2_gpu.h"
 
-extern "C" {
-int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
-unsigned fn2_grid_for( int64_t total, int blocksPerCU );
-int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream );
-int fn2_launch_selftest_math( int op, const float* in, float* out, int n, hipStream_t stream );
-}
+#include "fn2_launch.h"
 
 namespace fn2
 {
@@ -455,10 +450,10 @@ namespace fn2
         return f;
     }
 
-    bool DecodeRange( const int32_t* record, int stride, float& min, float& max )
+    bool DecodeRange( const int32_t* record, int slots, int stride, float& min, float& max )
     {
         int32_t kMin = 0x7fffffff, kMax = (int32_t)0x80000000u;
-        for( int s = 0; s < kRangeSlots; s++ )
+        for( int s = 0; s < slots; s++ )
         {
             kMin = std::min( kMin, record[s * stride] );
             kMax = std::max( kMax, record[s * stride + 1] );
@@ -543,6 +538,7 @@ namespace fn2
     bool FillSpecArgs( const fn2GenSpec& s, GenArgs& a, int64_t& total, std::string& err, const float* tileTable )
     {
         a.rangeStride = kRangeStrideSpread;   // single calls; the serving engine packs its records
+        a.rangeSlots = kCallRangeSlots;
         a.kind = s.kind;
         a.dims = s.dims;
         a.seed = s.seed;
@@ -682,7 +678,7 @@ namespace fn2
             }
         }
 
-        if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kRangeStrideSpread, c->stream ), "launch min/max init" ) ) return fail( tLastError );
+        if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kCallRangeSlots, kRangeStrideSpread, c->stream ), "launch min/max init" ) ) return fail( tLastError );
         if( !HipOk( LaunchProgram( a, dp, c->stream, nullptr ), "launch generator" ) ) return fail( tLastError );
         if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, kCallRangeInts * sizeof( int32_t ), hipMemcpyDeviceToHost, c->stream ), "read min/max" ) ) return fail( tLastError );
         if( !devOut )
@@ -698,7 +694,7 @@ namespace fn2
         }
         if( !HipOk( hipStreamSynchronize( c->stream ), "generator kernel" ) ) return fail( tLastError );
 
-        if( !DecodeRange( c->hKeys, kRangeStrideSpread, r.min, r.max ) )
+        if( !DecodeRange( c->hKeys, kCallRangeSlots, kRangeStrideSpread, r.min, r.max ) )
         {
             r.min = FastNoise::kInfinity;
             r.max = -FastNoise::kInfinity;
@@ -929,13 +925,13 @@ FASTNOISE_API int fn2GenerateDevice( const void* node, const fn2GenSpec* spec, f
     a.resultReg = dp->resultReg;
     a.out = dOut;
     a.minmax = outputMinMax ? c->dKeys : nullptr;
-    if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kRangeStrideSpread, st ), "launch min/max init" ) ) return 1;
+    if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kCallRangeSlots, kRangeStrideSpread, st ), "launch min/max init" ) ) return 1;
     if( !HipOk( LaunchProgram( a, dp, st, nullptr ), "launch generator" ) ) return 1;
     if( outputMinMax )
     {
         if( !HipOk( hipMemcpyAsync( c->hKeys, c->dKeys, kCallRangeInts * sizeof( int32_t ), hipMemcpyDeviceToHost, st ), "read min/max" ) ) return 1;
         if( !HipOk( hipStreamSynchronize( st ), "generator kernel" ) ) return 1;
-        if( !DecodeRange( c->hKeys, kRangeStrideSpread, outputMinMax[0], outputMinMax[1] ) )
+        if( !DecodeRange( c->hKeys, kCallRangeSlots, kRangeStrideSpread, outputMinMax[0], outputMinMax[1] ) )
         {
             outputMinMax[0] = FastNoise::kInfinity;
             outputMinMax[1] = -FastNoise::kInfinity;
@@ -973,14 +969,14 @@ FASTNOISE_API int fn2BenchGenerate( const void* node, const fn2GenSpec* spec, fl
     a.minmax = c->dKeys;
     for( int i = 0; i < warmup; i++ )
     {
-        if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kRangeStrideSpread, c->stream ), "launch" ) ) return 1;
+        if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kCallRangeSlots, kRangeStrideSpread, c->stream ), "launch" ) ) return 1;
         if( !HipOk( LaunchProgram( a, dp, c->stream, nullptr ), "launch" ) ) return 1;
     }
     if( !HipOk( hipStreamSynchronize( c->stream ), "warmup" ) ) return 1;
     // The range keys are reset once: every timed launch writes the same
     // output, so the keys accumulate the same min/max, and the timed region
     // holds generator launches only.
-    if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kRangeStrideSpread, c->stream ), "launch" ) ) return 1;
+    if( !HipOk( (hipError_t)fn2_launch_minmax_init( c->dKeys, 1, kCallRangeSlots, kRangeStrideSpread, c->stream ), "launch" ) ) return 1;
     if( !HipOk( hipStreamSynchronize( c->stream ), "range reset" ) ) return 1;
     auto t0 = std::chrono::steady_clock::now();
     if( !HipOk( hipEventRecord( c->ev0, c->stream ), "event" ) ) return 1;

@@ -79,9 +79,9 @@ namespace fn2
     float DecodeKey( int32_t k );
     // The range a record holds: false when no number was published to it
     // (every output NaN, or none), else its min and max
-    bool DecodeRange( const int32_t* record, int stride, float& min, float& max );
+    bool DecodeRange( const int32_t* record, int slots, int stride, float& min, float& max );
     // ints of a single call's range record
-    constexpr int kCallRangeInts = kRangeSlots * kRangeStrideSpread;
+    constexpr int kCallRangeInts = kCallRangeSlots * kRangeStrideSpread;
     // Launch arguments of a spec (everything but the program and the output).
     // A tileable spec uses `tileTable` (device floats laid out as FillTileTable
     // writes them) when it is given, else the process's cached table of its

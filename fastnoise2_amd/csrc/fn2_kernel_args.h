@@ -8,15 +8,21 @@
 namespace fn2
 {
     // A range record: the output range of one call as ordered-int keys, kept
-    // in kRangeSlots {min, max} pairs `stride` ints apart. A block (or a
-    // wave) publishes to the slot of its index, so that its atomics do not
-    // queue behind every other block's on one address; the range is the
-    // smallest min key and the largest max key over the slots. Measured on
-    // MI355X (1 slot -> 16 compact slots -> 16 slots on their own 128-byte
-    // lines): Perlin 64^3 15.0 -> 9.1 -> 4.4 us, a 1 M-point 4D call 56 -> 41
-    // -> 40 us; single calls take the spread layout, the serving engine
-    // (which reads a record back per request) the compact one.
+    // in `slots` {min, max} pairs `stride` ints apart (GenArgs::rangeSlots,
+    // a power of two, and GenArgs::rangeStride). A block (or a wave)
+    // publishes to the slot of its index, so that its atomics do not queue
+    // behind every other block's on one address: device-wide atomics on one
+    // 128-byte line complete one after another, about 11 ns each on MI355X.
+    // The range is the smallest min key and the largest max key over the
+    // slots. Measured (1 slot -> 16 compact slots -> 16 slots on their own
+    // lines): Perlin 64^3 15.0 -> 9.1 -> 4.4 us, a 1 M-point 4D call 56 ->
+    // 41 -> 40 us; 512 blocks storing 1 MB took 3.22 us with 16 lines, 2.69
+    // with 64 and 2.64 with 256 (2.22 without a record). Single calls take
+    // kCallRangeSlots spread slots; the serving engine, which reads a
+    // record back per request, kRangeSlots compact (or, persistent, spread)
+    // ones.
     constexpr int kRangeSlots = 16;
+    constexpr int kCallRangeSlots = 64;
     constexpr int kRangeStrideSpread = 32;    // a slot per 128-byte line
     constexpr int kRangeStrideCompact = 2;    // slots side by side
 
@@ -70,6 +76,7 @@ namespace fn2
         int32_t ppt;             // batched kernel: points per lane of each block of this job
         int32_t regs;            // registers the program uses (the interpreter's file holds as many)
         int32_t rangeStride;     // ints between the slots of the range record
+        int32_t rangeSlots;      // slots of the range record (a power of two)
         // index of this job's first point in its request: a request split
         // into several jobs evaluates points base + [0, total) of the whole
         // grid or image (position arrays are offset by the host instead)

@@ -25,9 +25,9 @@ namespace fn2k
     // BLOCK: the most threads a block of the kernel has (its launch bound);
     // the block may have fewer, any multiple of 64.
     template<int BLOCK>
-    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* record, int stride )
+    __device__ __forceinline__ void blockMinMax( float mn, float mx, int32_t* record, int stride, int slots )
     {
-        int32_t* keys = record + ( blockIdx.x % fn2::kRangeSlots ) * stride;
+        int32_t* keys = record + ( blockIdx.x & (uint32_t)( slots - 1 ) ) * stride;
         __shared__ float sMin[BLOCK / 64], sMax[BLOCK / 64];
 #pragma unroll
         for( int off = 32; off > 0; off >>= 1 )
@@ -183,7 +183,7 @@ namespace fn2k
                 emit( i, prog( a, R ) );
             }
         }
-        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride );
+        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride, a.rangeSlots );
     }
 
     // One tile of a batched request: points [first, first + threads * ppt)
@@ -235,7 +235,7 @@ namespace fn2k
                 }
             }
         }
-        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride );
+        if( a.minmax ) blockMinMax<BLOCK>( mn, mx, a.minmax, a.rangeStride, a.rangeSlots );
     }
 
     // Batched requests: block b runs tile (blockJob[b], blockBase[b]).
@@ -298,7 +298,6 @@ namespace fn2k
         const uint32_t items = nTiles * kWaves;
         const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane( (int)( blockIdx.x * kWaves + ( threadIdx.x >> 6 ) ) );
         const uint32_t waves = gridDim.x * kWaves;
-        const int slot = (int)( g % fn2::kRangeSlots );
         // this wave's part of the items beyond the first round: [first, end)
         const fn2::QueuePart q = fn2::queuePart( g, waves, items );
         const uint32_t first = q.first, end = q.end;
@@ -347,7 +346,7 @@ namespace fn2k
                     }
                 }
             }
-            waveMinMax( mn, mx, a.minmax, slot, a.rangeStride );
+            waveMinMax( mn, mx, a.minmax, (int)( g & (uint32_t)( a.rangeSlots - 1 ) ), a.rangeStride );
             if( first >= end ) break;   // no items in this wave's part
             const uint32_t pulled = atomicAdd( myCounter, lane == 0 ? 1u : 0u );
             w = first + (uint32_t)__builtin_amdgcn_readfirstlane( (int)pulled );

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include "fn2_kernel_body.h"
 #include "fn2_kernel_args.h"
+#include "fn2_launch.h"
 
 using namespace fn2d;
 using namespace fn2i;
@@ -114,14 +115,14 @@ __global__ void __launch_bounds__( 256 ) fn2_bloxel_faces( const float* __restri
     }
 }
 
-// Resets n consecutive range records (every slot, `stride` ints apart) and
-// zeroes nc work-queue counters
-__global__ void fn2_minmax_init( int32_t* keys, int n, int stride, uint32_t* counters, int nc )
+// Resets n consecutive range records (`slots` slots each, `stride` ints
+// apart) and zeroes nc work-queue counters
+__global__ void fn2_minmax_init( int32_t* keys, int n, int slots, int stride, uint32_t* counters, int nc )
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if( i < n * fn2::kRangeSlots )
+    if( i < n * slots )
     {
-        int32_t* k = keys + i * stride;   // slot i % kRangeSlots of record i / kRangeSlots
+        int32_t* k = keys + i * stride;   // slot i % slots of record i / slots
         k[0] = 0x7fffffff;          // key of +NaN region: above every float
         k[1] = (int32_t)0x80000000u;
     }
@@ -310,22 +311,23 @@ int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t*
     return (int)hipGetLastError();
 }
 
-int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream )
+int fn2_launch_minmax_init( int32_t* keys, int n, int slots, int stride, hipStream_t stream )
 {
     if( n <= 0 ) return 0;
-    if( stride < 2 ) return (int)hipErrorInvalidValue;
-    const int m = n * fn2::kRangeSlots;
-    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, stride, (uint32_t*)nullptr, 0 );
+    if( stride < 2 || slots < 1 || ( slots & ( slots - 1 ) ) != 0 ) return (int)hipErrorInvalidValue;
+    const int m = n * slots;
+    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, slots, stride, (uint32_t*)nullptr, 0 );
     return (int)hipGetLastError();
 }
 
 // The same, also zeroing nc work-queue counters (one launch per batch)
-int fn2_launch_batch_init( int32_t* keys, int n, int stride, uint32_t* counters, int nc, hipStream_t stream )
+int fn2_launch_batch_init( int32_t* keys, int n, int slots, int stride, uint32_t* counters, int nc, hipStream_t stream )
 {
-    if( n < 0 || nc < 0 || stride < 2 || ( nc > 0 && !counters ) ) return (int)hipErrorInvalidValue;
-    const int m = std::max( n * fn2::kRangeSlots, nc );
+    if( n < 0 || nc < 0 || stride < 2 || slots < 1 || ( slots & ( slots - 1 ) ) != 0 || ( nc > 0 && !counters ) )
+        return (int)hipErrorInvalidValue;
+    const int m = std::max( n * slots, nc );
     if( m == 0 ) return 0;
-    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, stride, counters, nc );
+    hipLaunchKernelGGL( fn2_minmax_init, dim3( ( m + 63 ) / 64 ), dim3( 64 ), 0, stream, keys, n, slots, stride, counters, nc );
     return (int)hipGetLastError();
 }
 

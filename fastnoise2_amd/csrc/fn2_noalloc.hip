@@ -29,9 +29,7 @@
 #include "fn2_jit.h"
 #include <FastNoise/FastNoise.h>
 
-extern "C" int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
-extern "C" unsigned fn2_grid_for( int64_t total, int blocksPerCU );
-extern "C" int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream );
+#include "fn2_launch.h"
 
 namespace
 {
@@ -186,7 +184,7 @@ FASTNOISE_API int fn2GenerateDeviceNoAlloc( const void* node, const fn2GenSpec* 
     a.minmax = outputMinMax ? dKeys : nullptr;
     // the range record starts empty: a fill kernel, not a memset, so that the
     // call can be recorded into a graph
-    if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( dKeys, 1, kRangeStrideSpread, st ), "launch min/max init" ) ) return 1;
+    if( outputMinMax && !HipOk( (hipError_t)fn2_launch_minmax_init( dKeys, 1, kCallRangeSlots, kRangeStrideSpread, st ), "launch min/max init" ) ) return 1;
     if( jit )
     {
         GenArgs copy = a;
@@ -201,7 +199,7 @@ FASTNOISE_API int fn2GenerateDeviceNoAlloc( const void* node, const fn2GenSpec* 
         int32_t keys[kCallRangeInts];
         if( !HipOk( hipMemcpyAsync( keys, dKeys, sizeof( keys ), hipMemcpyDeviceToHost, st ), "read min/max" ) ) return 1;
         if( !HipOk( hipStreamSynchronize( st ), "generator kernel" ) ) return 1;
-        if( !DecodeRange( keys, kRangeStrideSpread, outputMinMax[0], outputMinMax[1] ) )
+        if( !DecodeRange( keys, kCallRangeSlots, kRangeStrideSpread, outputMinMax[0], outputMinMax[1] ) )
         {
             outputMinMax[0] = FastNoise::kInfinity;
             outputMinMax[1] = -FastNoise::kInfinity;

@@ -48,16 +48,7 @@
 #include "fn2_internal.h"
 #include "fn2_jit.h"
 
-extern "C" {
-int fn2_launch_generate( const fn2::GenArgs* args, int dimMask, int featMask, hipStream_t stream );
-int fn2_launch_generate_batch( const fn2::GenArgs* dJobs, const int32_t* dBlockJob, const uint32_t* dBlockBase, int nBlocks, int dimMask, int featMask,
-                               int regs, hipStream_t stream );
-int fn2_launch_minmax_init( int32_t* keys, int n, int stride, hipStream_t stream );
-int fn2_launch_generate_queue( const fn2::GenArgs* dJobs, const int32_t* dTileJob, const uint32_t* dTileBase, int nTiles, int dimMask, int featMask,
-                               int regs, uint32_t* dCounter, hipStream_t stream );
-int fn2_launch_batch_init( int32_t* keys, int n, int stride, uint32_t* counters, int nc, hipStream_t stream );
-int fn2_batch_block_threads( void );
-}
+#include "fn2_launch.h"
 
 namespace
 {
@@ -681,6 +672,7 @@ namespace
             a.out = tuneOut;
             a.minmax = tuneKeys;
             a.rangeStride = fn2::kRangeStrideCompact;
+            a.rangeSlots = fn2::kRangeSlots;
             return true;
         }
 
@@ -710,7 +702,7 @@ namespace
                 const int32_t* bjp = dBJ;
                 const uint32_t* bbp = dBB;
                 void* params[] = { (void*)&jobs, (void*)&bjp, (void*)&bbp };
-                if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, fn2::kRangeStrideCompact, tuneStream ), "tune launch" ) ||
+                if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( tuneKeys, 1, fn2::kRangeSlots, fn2::kRangeStrideCompact, tuneStream ), "tune launch" ) ||
                     !fn2::HipOk( hipEventRecord( tuneEv0, tuneStream ), "tune event" ) ||
                     !fn2::HipOk( jit ? hipModuleLaunchKernel( jit->batch, (unsigned)nb, 1, 1, (unsigned)threads, 1, 1, 0, tuneStream, params, nullptr )
                                      : (hipError_t)fn2_launch_generate_batch( tuneJob, dBJ, dBB, nb, prog->dimMask, prog->featMask, prog->regs, tuneStream ),
@@ -826,6 +818,7 @@ namespace
                 a.ppt = ppt;
                 a.minmax = s.dKeys + (size_t)jobRangeInts * j;
                 a.rangeStride = jobStride;
+                a.rangeSlots = fn2::kRangeSlots;
                 hJobs[j] = a;
                 const int64_t tile = (int64_t)r->threads * ppt;
                 int rb = (int)( ( r->points + tile - 1 ) / tile );
@@ -916,10 +909,10 @@ namespace
             if( mode == FN2_SERVE_PERSISTENT )
             {
                 // range keys and one zero counter per work-queue launch, in one kernel
-                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, jobStride, s.dCounter, (int)shape.size() * fn2::kQueueCounterInts, s.stream ), "range reset" ) )
+                if( !fn2::HipOk( (hipError_t)fn2_launch_batch_init( s.dKeys, nJobs, fn2::kRangeSlots, jobStride, s.dCounter, (int)shape.size() * fn2::kQueueCounterInts, s.stream ), "range reset" ) )
                     return false;
             }
-            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, jobStride, s.stream ), "range reset" ) ) return false;
+            else if( !fn2::HipOk( (hipError_t)fn2_launch_minmax_init( s.dKeys, nJobs, fn2::kRangeSlots, jobStride, s.stream ), "range reset" ) ) return false;
             int first = 0;
             for( size_t gi = 0; gi < shape.size(); gi++ )
             {
@@ -1042,7 +1035,7 @@ namespace
                     copies->Run( segs );
                     for( Request* r : s.reqs )
                     {
-                        if( !fn2::DecodeRange( s.hKeys + (size_t)r->job * jobRangeInts, jobStride, r->mm[0], r->mm[1] ) )
+                        if( !fn2::DecodeRange( s.hKeys + (size_t)r->job * jobRangeInts, fn2::kRangeSlots, jobStride, r->mm[0], r->mm[1] ) )
                         {
                             r->mm[0] = __builtin_inff();
                             r->mm[1] = -__builtin_inff();

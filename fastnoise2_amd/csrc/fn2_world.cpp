@@ -29,7 +29,7 @@
 #include "fn2_backend.h"
 #include "fn2_gpu.h"
 
-extern "C" int fn2_launch_bloxel_faces( const float* density, int gen, float iso, uint32_t* counts, hipStream_t stream );
+#include "fn2_launch.h"
 
 namespace
 {
