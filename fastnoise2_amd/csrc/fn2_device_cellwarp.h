@@ -295,6 +295,157 @@ namespace fn2d
         for( int k = 0; k < 4; k++ ) dist[k] = ibits( (int32_t)key[k] );
     }
 
+#if defined( __HIP_DEVICE_COMPILE__ )
+    // Per-wave table of the table walk (cellularDistancesTable): for each
+    // cell of a 5^D box, its jitter vector and 1 / |vector|, one region per
+    // wave of a block of at most 256 threads.
+    struct alignas( 16 ) CellEntry
+    {
+        float v[3];      // jitter vector (the third is 0 in 2D)
+        float invMag;    // 1 / |vector| (each lane scales it by its own jitter)
+    };
+    template<int D>
+    __device__ __forceinline__ CellEntry* cellTable()
+    {
+        static_assert( D == 2 || D == 3, "cell table: 2D or 3D" );
+        __shared__ CellEntry waves[4][D == 2 ? 25 : 125];
+        return waves[threadIdx.x >> 6];
+    }
+
+    // CellularDistance, squared distances, no size jitter, 2D/3D: the pruned
+    // walk with the per-cell work shared by the wave. When every active
+    // lane's walk starts within one cell of the first active lane's on every
+    // axis, the 3^D cells of all lanes lie in a box of 5^D cells, and the
+    // lanes fill the wave's table with each box cell's hash-derived jitter
+    // vector and inverse length (the reference's per-cell work,
+    // Cellular.inl:391-409, computed once per wave instead of once per lane
+    // and cell). Each lane then walks its own cells as the pruned walk does,
+    // reading the vector and the inverse length from the table and scaling
+    // the length by its own jitter (a node may set it per point); its
+    // distance is the same fused multiply-add of the same operands, so the
+    // slots end bit for bit as the pruned walk's. Returns false, having done
+    // nothing, for a wave whose lanes lie further apart (scattered positions)
+    // or differ in seed.
+    template<int D, int SLOTS>
+    __device__ __forceinline__ bool cellularDistancesTable( const CellParams& cp, const float* pos, float* dist )
+    {
+        using Order = MirroredCellOrder<D>;
+        constexpr Order order;
+        constexpr int kEntries = D == 2 ? 25 : 125;
+        const int32_t prime[3] = { kPX, kPY, kPZ };
+        int32_t c[D], first[D];
+        bool near = cp.seed == __builtin_amdgcn_readfirstlane( cp.seed );
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            c[a] = iadd( roundToInt( pos[a] ), -1 );
+            first[a] = __builtin_amdgcn_readfirstlane( c[a] );
+            // c - first in {-1, 0, 1}, in wrapping arithmetic
+            near = near && (uint32_t)c[a] - (uint32_t)first[a] + 1u <= 2u;
+        }
+        if( !waveAll( near ) ) return false;
+
+        // the box starts one cell below the first lane's walk: table
+        // coordinate t of axis a is cell first[a] - 1 + t, t in 0..4
+        CellEntry* tbl = cellTable<D>();
+        const uint64_t active = __ballot( 1 );
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi( (uint32_t)( active >> 32 ), __builtin_amdgcn_mbcnt_lo( (uint32_t)active, 0u ) );
+        const uint32_t lanes = (uint32_t)__builtin_popcountll( active );
+        // the previous use of the table by this wave is over
+        __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        for( uint32_t e = rank; e < (uint32_t)kEntries; e += lanes )
+        {
+            int32_t hc[D];
+            uint32_t rest = e;
+#pragma unroll
+            for( int a = 0; a < D; a++ )
+            {
+                const int32_t t = (int32_t)( rest % 5u );
+                rest /= 5u;
+                hc[a] = imul( iadd( first[a], t - 1 ), prime[a] );
+            }
+            float v[D];
+            cellVector<D>( cellHash<D>( cp.seed, hc ), v );
+            const float invMag = invSqrtCell<D>( cellMagSq<D>( v ) );
+            CellEntry entry;
+            entry.v[0] = v[0];
+            entry.v[1] = v[1];
+            entry.v[2] = D > 2 ? v[D - 1] : 0.0f;
+            entry.invMag = invMag;
+            tbl[e] = entry;
+        }
+        __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+
+        // the lane's mirrored columns: table offsets and cell positions
+        int32_t T[D][3];
+        float rel[D][3];
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            const int32_t scale = a == 0 ? 1 : a == 1 ? 5 : 25;
+            const int32_t t0 = (int32_t)( (uint32_t)c[a] - (uint32_t)first[a] + 1u );   // 0..2
+            T[a][0] = t0 * scale;
+            T[a][1] = T[a][0] + scale;
+            T[a][2] = T[a][1] + scale;
+            float fc = (float)c[a];
+            rel[a][0] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][1] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][2] = fc - pos[a];
+            const bool below = rel[a][1] > 0.0f;
+            const int32_t i0 = T[a][0], i2 = T[a][2];
+            const float r0 = rel[a][0], r2 = rel[a][2];
+            T[a][0] = below ? i2 : i0;
+            T[a][2] = below ? i0 : i2;
+            rel[a][0] = below ? r2 : r0;
+            rel[a][2] = below ? r0 : r2;
+        }
+        uint32_t key[4] = { 0x7f800000u, 0x7f800000u, 0x7f800000u, 0x7f800000u };   // +inf
+        const float jitterBound = fabs_( cp.jitter ) * ( 1.0f + 0x1p-16f );
+        float limit = __builtin_inff();
+#pragma unroll
+        for( int k = 0; k < Order::kCells; k++ )
+        {
+            const int code = order.e[k], cls = code >> 8;
+            int o[D];
+#pragma unroll
+            for( int a = 0; a < D; a++ ) o[a] = ( code >> ( 2 * a ) ) & 3;
+            if( k >= SLOTS )
+            {
+                if( ( order.e[k - 1] >> 8 ) != cls )
+                {
+                    const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+                    limit = r * r;
+                }
+                float pp = rel[0][o[0]] * rel[0][o[0]];
+#pragma unroll
+                for( int a = 1; a < D; a++ ) pp = fmadd( rel[a][o[a]], rel[a][o[a]], pp );
+                if( waveAll( pp > limit ) ) continue;
+            }
+            int32_t idx = T[0][o[0]];
+#pragma unroll
+            for( int a = 1; a < D; a++ ) idx += T[a][o[a]];
+            const CellEntry cell = tbl[idx];
+            const float invMag = cp.jitter * cell.invMag;
+            float d[D];
+#pragma unroll
+            for( int a = 0; a < D; a++ ) d[a] = fmadd( cell.v[a], invMag, rel[a][o[a]] );
+            const float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );
+            const uint32_t nb = (uint32_t)fbits( nd );
+            if constexpr( SLOTS > 3 ) key[3] = umax( umin( key[3], nb ), key[2] );
+            if constexpr( SLOTS > 2 ) key[2] = umax( umin( key[2], nb ), key[1] );
+            if constexpr( SLOTS > 1 ) key[1] = umax( umin( key[1], nb ), key[0] );
+            key[0] = umin( key[0], nb );
+        }
+#pragma unroll
+        for( int k = 0; k < 4; k++ ) dist[k] = ibits( (int32_t)key[k] );
+        return true;
+    }
+#endif
+
 #ifndef FN2_SHARED_CELL_WALK
 #define FN2_SHARED_CELL_WALK 0
 #endif
@@ -527,13 +678,19 @@ namespace fn2d
     // kernel is compiled) may keep just those: they hold the same values.
     // PRUNED: take the pruned walk where it applies (specialised kernels; the
     // interpreter keeps the one walk, whose registers it can afford)
-    template<int D, int SLOTS = 4, bool PRUNED = false>
+    // TABLE: try the table walk first (specialised kernels: its table is
+    // static LDS, which the interpreter's register file would lose)
+    template<int D, int SLOTS = 4, bool PRUNED = false, bool TABLE = false>
     FN2D void cellularDistances( const CellParams& cp, const float* pos, float* dist )
     {
         if constexpr( PRUNED )
         {
             if( squaredDistance( cp.distFunc ) && !cp.sizeJitterOn )
             {
+#if defined( __HIP_DEVICE_COMPILE__ )
+                if constexpr( TABLE && ( D == 2 || D == 3 ) )
+                    if( cellularDistancesTable<D, SLOTS>( cp, pos, dist ) ) return;
+#endif
 #if defined( __HIP_DEVICE_COMPILE__ ) && FN2_SHARED_CELL_WALK
                 if constexpr( D == 3 && SLOTS <= 2 )
                 {

@@ -17,6 +17,14 @@
 // (on MI355X the warped cellular tree's interpreter kernel went from 0.633 to
 // 0.461 ms on a 256^3 grid and from 0.672 to 0.563 ms on 16.8 M positions,
 // with the same bits); defined false, it takes the full walk
+// Specialised kernels share the per-cell work of a coherent wave through
+// an LDS table (cellularDistancesTable); defined 0, they take the pruned
+// walk. The position-array variant with the shared far-cell walk leaves the
+// table out: its points are scattered, and the two LDS regions together
+// would cost resident blocks.
+#ifndef FN2_CELL_TABLE
+#define FN2_CELL_TABLE 1
+#endif
 #ifndef FN2_PRUNED_CELL_WALK
 #define FN2_PRUNED_CELL_WALK true
 #endif
@@ -216,7 +224,7 @@ namespace fn2i
             if( in.op == fn2::OP_CELL_DISTANCE )
             {
                 float dist[4];
-                cellularDistances<D, SLOTS, ( OP >= 0 ) || FN2_PRUNED_CELL_WALK>( cp, pos, dist );
+                cellularDistances<D, SLOTS, ( OP >= 0 ) || FN2_PRUNED_CELL_WALK, ( OP >= 0 ) && FN2_CELL_TABLE && !FN2_SHARED_CELL_WALK>( cp, pos, dist );
                 R[in.r[0]] = scaleOutput( cellularReturn( dist, in.p[0], in.p[1], in.p[2], in.p[3], 0.0f ), K[1], K[2], K[4] );
                 return;
             }
