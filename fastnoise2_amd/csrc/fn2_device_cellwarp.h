@@ -209,8 +209,9 @@ namespace fn2d
     // cell that cannot change the kept slots. A cell's feature point lies
     // within |jitter| of its lattice point, so its distance is at least
     // |lattice point - position| - |jitter|; when that exceeds the largest kept
-    // distance (with margins of 2^-10, far above the rounding of either side)
-    // the insertion network would leave every kept slot as it is. The kept
+    // distance (with margins of 2^-10, far above the rounding of either side
+    // and the 1 ulp of the hardware square root the bound takes) the
+    // insertion network would leave every kept slot as it is. The kept
     // slots are the k smallest distances, whatever the order cells come in, so
     // the result is the one of the full walk.
     template<int D, int SLOTS>
@@ -262,7 +263,7 @@ namespace fn2d
                 if( ( order.e[k - 1] >> 8 ) != cls )
                 {
                     // squared bound beyond which a cell cannot reach the kept slots
-                    const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+                    const float r = sqrtBound( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
                     limit = r * r;
                 }
                 float pp = rel[0][o[0]] * rel[0][o[0]];
@@ -417,7 +418,7 @@ namespace fn2d
             {
                 if( ( order.e[k - 1] >> 8 ) != cls )
                 {
-                    const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+                    const float r = sqrtBound( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
                     limit = r * r;
                 }
                 float pp = rel[0][o[0]] * rel[0][o[0]];
@@ -561,7 +562,7 @@ namespace fn2d
         }
         // the far cells this lane can still need, as a mask over cell codes
         const float jitterBound = fabs_( cp.jitter ) * ( 1.0f + 0x1p-16f );
-        const float r = __builtin_sqrtf( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
+        const float r = sqrtBound( ibits( (int32_t)key[SLOTS - 1] ) * ( 1.0f + 0x1p-10f ) ) + jitterBound + 0x1p-10f;
         const float limit = r * r;
         // `need` per lane, `any` their union over the wave's active lanes
         // (wave-uniform), both as masks over cell codes

@@ -125,6 +125,15 @@ namespace fn2d
     FN2D float recipPosRN( float b ) { return 1.0f / b; }
     FN2D float sqrtPosRNFromBelow( float x ) { return __builtin_sqrtf( x ); }
 #endif
+    // Square root for bounds that carry margins far above 1 ulp (the
+    // cellular walks' pruning limits): on the device the hardware estimate
+    // (v_sqrt_f32, within 1 ulp), one instruction instead of the correctly
+    // rounded routine's range scaling and correction steps
+#if defined( __HIP_DEVICE_COMPILE__ )
+    FN2D float sqrtBound( float x ) { return __builtin_amdgcn_sqrtf( x ); }
+#else
+    FN2D float sqrtBound( float x ) { return __builtin_sqrtf( x ); }
+#endif
     FN2D uint32_t umin( uint32_t a, uint32_t b ) { return a < b ? a : b; }
     FN2D uint32_t umax( uint32_t a, uint32_t b ) { return a > b ? a : b; }
 
