@@ -313,6 +313,71 @@ namespace fn2d
         return waves[threadIdx.x >> 6];
     }
 
+    // each cell's hash, beside the table (CellularValue)
+    template<int D>
+    __device__ __forceinline__ int32_t* cellHashTable()
+    {
+        __shared__ int32_t waves[4][D == 2 ? 25 : 125];
+        return waves[threadIdx.x >> 6];
+    }
+
+    // The table walks' common part. c[a]: the lane's first walk cell on
+    // axis a (the rounded position - 1); first[a]: the first active lane's.
+    // When every active lane has c within one of first on every axis and
+    // the same seed, fills the wave's table for the box of cells first - 1
+    // .. first + 3 (table coordinate t = cell - first + 1) and returns true;
+    // else returns false, having written nothing. HASH: also keep each
+    // cell's hash (CellularValue returns it).
+    template<int D, bool HASH>
+    __device__ __forceinline__ bool fillCellTable( const CellParams& cp, const float* pos, int32_t* c, int32_t* first )
+    {
+        constexpr int kEntries = D == 2 ? 25 : 125;
+        const int32_t prime[3] = { kPX, kPY, kPZ };
+        bool near = cp.seed == __builtin_amdgcn_readfirstlane( cp.seed );
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            c[a] = iadd( roundToInt( pos[a] ), -1 );
+            first[a] = __builtin_amdgcn_readfirstlane( c[a] );
+            // c - first in {-1, 0, 1}, in wrapping arithmetic
+            near = near && (uint32_t)c[a] - (uint32_t)first[a] + 1u <= 2u;
+        }
+        if( !waveAll( near ) ) return false;
+
+        CellEntry* tbl = cellTable<D>();
+        const uint64_t active = __ballot( 1 );
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi( (uint32_t)( active >> 32 ), __builtin_amdgcn_mbcnt_lo( (uint32_t)active, 0u ) );
+        const uint32_t lanes = (uint32_t)__builtin_popcountll( active );
+        // the previous use of the table by this wave is over
+        __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        for( uint32_t e = rank; e < (uint32_t)kEntries; e += lanes )
+        {
+            int32_t hc[D];
+            uint32_t rest = e;
+#pragma unroll
+            for( int a = 0; a < D; a++ )
+            {
+                const int32_t t = (int32_t)( rest % 5u );
+                rest /= 5u;
+                hc[a] = imul( iadd( first[a], t - 1 ), prime[a] );
+            }
+            const int32_t h = cellHash<D>( cp.seed, hc );
+            float v[D];
+            cellVector<D>( h, v );
+            CellEntry entry;
+            entry.v[0] = v[0];
+            entry.v[1] = v[1];
+            entry.v[2] = D > 2 ? v[D - 1] : 0.0f;
+            entry.invMag = invSqrtCell<D>( cellMagSq<D>( v ) );
+            tbl[e] = entry;
+            if constexpr( HASH ) cellHashTable<D>()[e] = h;
+        }
+        __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        return true;
+    }
+
     // CellularDistance, squared distances, no size jitter, 2D/3D: the pruned
     // walk with the per-cell work shared by the wave. When every active
     // lane's walk starts within one cell of the first active lane's on every
@@ -332,52 +397,9 @@ namespace fn2d
     {
         using Order = MirroredCellOrder<D>;
         constexpr Order order;
-        constexpr int kEntries = D == 2 ? 25 : 125;
-        const int32_t prime[3] = { kPX, kPY, kPZ };
         int32_t c[D], first[D];
-        bool near = cp.seed == __builtin_amdgcn_readfirstlane( cp.seed );
-#pragma unroll
-        for( int a = 0; a < D; a++ )
-        {
-            c[a] = iadd( roundToInt( pos[a] ), -1 );
-            first[a] = __builtin_amdgcn_readfirstlane( c[a] );
-            // c - first in {-1, 0, 1}, in wrapping arithmetic
-            near = near && (uint32_t)c[a] - (uint32_t)first[a] + 1u <= 2u;
-        }
-        if( !waveAll( near ) ) return false;
-
-        // the box starts one cell below the first lane's walk: table
-        // coordinate t of axis a is cell first[a] - 1 + t, t in 0..4
-        CellEntry* tbl = cellTable<D>();
-        const uint64_t active = __ballot( 1 );
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi( (uint32_t)( active >> 32 ), __builtin_amdgcn_mbcnt_lo( (uint32_t)active, 0u ) );
-        const uint32_t lanes = (uint32_t)__builtin_popcountll( active );
-        // the previous use of the table by this wave is over
-        __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
-        __builtin_amdgcn_wave_barrier();
-        for( uint32_t e = rank; e < (uint32_t)kEntries; e += lanes )
-        {
-            int32_t hc[D];
-            uint32_t rest = e;
-#pragma unroll
-            for( int a = 0; a < D; a++ )
-            {
-                const int32_t t = (int32_t)( rest % 5u );
-                rest /= 5u;
-                hc[a] = imul( iadd( first[a], t - 1 ), prime[a] );
-            }
-            float v[D];
-            cellVector<D>( cellHash<D>( cp.seed, hc ), v );
-            const float invMag = invSqrtCell<D>( cellMagSq<D>( v ) );
-            CellEntry entry;
-            entry.v[0] = v[0];
-            entry.v[1] = v[1];
-            entry.v[2] = D > 2 ? v[D - 1] : 0.0f;
-            entry.invMag = invMag;
-            tbl[e] = entry;
-        }
-        __builtin_amdgcn_fence( __ATOMIC_ACQ_REL, "wavefront" );
-        __builtin_amdgcn_wave_barrier();
+        if( !fillCellTable<D, false>( cp, pos, c, first ) ) return false;
+        const CellEntry* tbl = cellTable<D>();
 
         // the lane's mirrored columns: table offsets and cell positions
         int32_t T[D][3];
@@ -443,6 +465,71 @@ namespace fn2d
         }
 #pragma unroll
         for( int k = 0; k < 4; k++ ) dist[k] = ibits( (int32_t)key[k] );
+        return true;
+    }
+
+    // CellularValue (Cellular.inl:17-292) on the wave's cell table: the
+    // reference's walk, x outermost, over the same cells with the same
+    // running offsets, since the value of tied distances depends on the
+    // order (the first of equal distances stays); only each cell's hash,
+    // jitter vector and inverse length come from the table. Returns false,
+    // having done nothing, where fillCellTable does.
+    template<int D>
+    __device__ __forceinline__ bool cellularValueTable( const CellParams& cp, const float* pos, int valueIndex, float& result )
+    {
+        int32_t c[D], first[D];
+        if( !fillCellTable<D, true>( cp, pos, c, first ) ) return false;
+        const CellEntry* tbl = cellTable<D>();
+        const int32_t* htbl = cellHashTable<D>();
+        int32_t T[D];
+        float F[D][3];
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            const int32_t scale = a == 0 ? 1 : a == 1 ? 5 : 25;
+            T[a] = (int32_t)( (uint32_t)c[a] - (uint32_t)first[a] + 1u ) * scale;   // the lane's first cell
+            F[a][0] = (float)c[a] - pos[a];
+            F[a][1] = F[a][0] + 1.0f;
+            F[a][2] = F[a][1] + 1.0f;
+        }
+        float dist[4] = { __builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff() };
+        int32_t vh[4] = { 0, 0, 0, 0 };
+#pragma unroll
+        for( int i0 = 0; i0 < 3; i0++ )
+#pragma unroll
+            for( int i1 = 0; i1 < 3; i1++ )
+#pragma unroll
+                for( int i2 = 0; i2 < ( D > 2 ? 3 : 1 ); i2++ )
+                {
+                    int32_t idx = T[0] + i0 + T[1] + 5 * i1;
+                    if constexpr( D > 2 ) idx += T[D - 1] + 25 * i2;
+                    const CellEntry cell = tbl[idx];
+                    const int32_t h = htbl[idx];
+                    const float invMag = cp.jitter * cell.invMag;
+                    const int ii[3] = { i0, i1, i2 };
+                    float d[D];
+#pragma unroll
+                    for( int a = 0; a < D; a++ ) d[a] = fmadd( cell.v[a], invMag, F[a][ii[a]] );
+                    float nd = calcDistance<D>( cp.distFunc, D == 2, d, cp.minkowskiP );
+                    if( cp.sizeJitterOn ) nd *= fnmadd( cp.sizeJitter, sizeJitterBits<D>( h ), 1.0f );
+                    int32_t nh = h;
+#pragma unroll
+                    for( int i = 0; i < 4; i++ )
+                    {
+                        if( i > valueIndex ) break;
+                        bool closer = nd < dist[i];
+                        float ld = dist[i];
+                        int32_t lh = vh[i];
+                        dist[i] = closer ? nd : dist[i];
+                        vh[i] = closer ? nh : vh[i];
+                        nd = closer ? ld : nd;
+                        nh = closer ? lh : nh;
+                    }
+                }
+        int32_t sel = vh[0];
+#pragma unroll
+        for( int i = 1; i < 4; i++ ) sel = valueIndex == i ? vh[i] : sel;
+        result = (float)sel;
         return true;
     }
 #endif

@@ -217,6 +217,17 @@ namespace fn2i
         if constexpr( opIn<OP, fn2::OP_CELL_VALUE>() )
             if( in.op == fn2::OP_CELL_VALUE )
             {
+#if defined( __HIP_DEVICE_COMPILE__ )
+                if constexpr( ( OP >= 0 ) && FN2_CELL_TABLE && !FN2_SHARED_CELL_WALK && ( D == 2 || D == 3 ) )
+                {
+                    float v;
+                    if( cellularValueTable<D>( cp, pos, in.p[1], v ) )
+                    {
+                        R[in.r[0]] = scaleOutput( v, K[1], K[2], K[4] );
+                        return;
+                    }
+                }
+#endif
                 R[in.r[0]] = scaleOutput( cellularValue<D>( cp, pos, in.p[1] ), K[1], K[2], K[4] );
                 return;
             }

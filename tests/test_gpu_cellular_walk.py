@@ -81,3 +81,36 @@ def test_huge_and_special_coordinates(gpu):
     out, _ = node.gen_position_array([xs, ys, zs], seed=5)
     ref, _ = oracle.Tree(spec.encode()).positions([xs, ys, zs], seed=5)
     assert_bitexact(out, ref, "huge and special coordinates")
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("jitter", range(len(JITTERS)), ids=["j1", "jneg", "j0", "jsmall", "jbig", "jnode"])
+def test_value_table_walk_matches_oracle(gpu, dims, jitter):
+    # CellularValue in a specialised kernel reads each cell's hash, jitter
+    # vector and inverse length from the wave's LDS table when the wave's
+    # cells lie close together (grids, clustered positions), and walks in
+    # the reference's order, which decides between tied distances (jitter 0);
+    # scattered positions take the per-lane walk. Per-point jitter and size
+    # jitter come from nodes, so lanes of one wave scale the same table
+    # entries differently.
+    size_jitter = S("Value", feature_scale=5.0) if jitter == 5 else 0.25
+    for vi, func in [(0, "Euclidean"), (2, "Manhattan"), (1, "Hybrid")]:
+        spec = S("CellularValue", feature_scale=9.0, distance_function=func, value_index=vi,
+                 grid_jitter=JITTERS[jitter], size_jitter=size_jitter)
+        node = fn.Node.from_encoded(spec.encode())
+        cnt = {2: [97, 41], 3: [61, 13, 10]}[dims]
+        off = [-31.25, 7.5, 100.0][:dims]
+        gen = {2: node.gen_uniform_grid_2d, 3: node.gen_uniform_grid_3d}[dims]
+        out, mm = gen(*off, *cnt, *([0.37] * dims), seed=777)
+        assert node.jit_state(dims) == 1, fn.last_error()
+        ref, ref_mm = oracle.Tree(spec.encode()).grid(off, cnt, [0.37] * dims, seed=777)
+        assert_bitexact(out, ref, "%dD grid jitter %d value index %d %s" % (dims, jitter, vi, func))
+        assert mm == ref_mm
+        rng = np.random.default_rng(dims + 10 * jitter)
+        pos = rng.uniform(-300.0, 300.0, (dims, 3000)).astype(np.float32)
+        pos[:, :2000] = (rng.normal(0.0, 1.0, (dims, 2000)) * 2.0 + 40.0).astype(np.float32)   # clustered
+        pos[:, 500:600] = (np.round(pos[:, 500:600] / 9.0) * 9.0).astype(np.float32)          # on lattice points
+        out, mm = node.gen_position_array(list(pos), seed=777)
+        ref, ref_mm = oracle.Tree(spec.encode()).positions(list(pos), seed=777)
+        assert_bitexact(out, ref, "%dD positions jitter %d value index %d %s" % (dims, jitter, vi, func))
+        assert mm == ref_mm
