@@ -468,6 +468,110 @@ namespace fn2d
         return true;
     }
 
+    // CellularDistance where the pruned walk does not apply (distance
+    // functions other than the squared ones, size jitter) on the wave's cell
+    // table: the full walk of cellularDistances, same order and the same
+    // float min / max steps, with each cell's hash, jitter vector and inverse
+    // length from the table. Returns false, having done nothing, where
+    // fillCellTable does.
+    template<int D, int SLOTS>
+    __device__ __forceinline__ bool cellularDistancesTableAll( const CellParams& cp, const float* pos, float* dist )
+    {
+        int32_t c[D], first[D];
+        if( !fillCellTable<D, true>( cp, pos, c, first ) ) return false;
+        const CellEntry* tbl = cellTable<D>();
+        const int32_t* htbl = cellHashTable<D>();
+        int32_t T[D];
+        float rel[D][3];
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            const int32_t scale = a == 0 ? 1 : a == 1 ? 5 : 25;
+            T[a] = (int32_t)( (uint32_t)c[a] - (uint32_t)first[a] + 1u ) * scale;   // the lane's first cell
+            float fc = (float)c[a];
+            rel[a][0] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][1] = fc - pos[a];
+            fc = fc + 1.0f;
+            rel[a][2] = fc - pos[a];
+        }
+        dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
+#pragma unroll
+        for( int i0 = 0; i0 < 3; i0++ )
+#pragma unroll
+            for( int i1 = 0; i1 < 3; i1++ )
+#pragma unroll
+                for( int i2 = 0; i2 < ( D > 2 ? 3 : 1 ); i2++ )
+                {
+                    int32_t idx = T[0] + i0 + T[1] + 5 * i1;
+                    if constexpr( D > 2 ) idx += T[D - 1] + 25 * i2;
+                    const CellEntry cell = tbl[idx];
+                    const float invMag = cp.jitter * cell.invMag;
+                    const int ii[3] = { i0, i1, i2 };
+                    float d[D];
+#pragma unroll
+                    for( int a = 0; a < D; a++ ) d[a] = fmadd( cell.v[a], invMag, rel[a][ii[a]] );
+                    float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );
+                    if( cp.sizeJitterOn ) nd *= fnmadd( cp.sizeJitter, sizeJitterBits<D>( htbl[idx] ), 1.0f );
+                    if constexpr( SLOTS > 3 ) dist[3] = fmaxf_( fminf_( dist[3], nd ), dist[2] );
+                    if constexpr( SLOTS > 2 ) dist[2] = fmaxf_( fminf_( dist[2], nd ), dist[1] );
+                    if constexpr( SLOTS > 1 ) dist[1] = fmaxf_( fminf_( dist[1], nd ), dist[0] );
+                    dist[0] = fminf_( dist[0], nd );
+                }
+        return true;
+    }
+
+    // CellularLookup's closest cell (Cellular.inl:568-807) on the wave's
+    // cell table: the walk of cellularLookupPos, same order (the first of
+    // equal distances stays) and steps, with each cell's hash, jitter vector
+    // and inverse length from the table. Returns false, having done nothing,
+    // where fillCellTable does.
+    template<int D>
+    __device__ __forceinline__ bool cellularLookupTable( const CellParams& cp, const float* pos, float* cellPos )
+    {
+        int32_t c[D], first[D];
+        if( !fillCellTable<D, true>( cp, pos, c, first ) ) return false;
+        const CellEntry* tbl = cellTable<D>();
+        const int32_t* htbl = cellHashTable<D>();
+        int32_t T[D];
+        float F[D][3];
+#pragma unroll
+        for( int a = 0; a < D; a++ )
+        {
+            const int32_t scale = a == 0 ? 1 : a == 1 ? 5 : 25;
+            T[a] = (int32_t)( (uint32_t)c[a] - (uint32_t)first[a] + 1u ) * scale;   // the lane's first cell
+            F[a][0] = (float)c[a];
+            F[a][1] = F[a][0] + 1.0f;
+            F[a][2] = F[a][1] + 1.0f;
+        }
+        float distance = 3.40282347e+38f;
+#pragma unroll
+        for( int a = 0; a < D; a++ ) cellPos[a] = 0.0f;
+#pragma unroll
+        for( int i0 = 0; i0 < 3; i0++ )
+#pragma unroll
+            for( int i1 = 0; i1 < 3; i1++ )
+#pragma unroll
+                for( int i2 = 0; i2 < ( D > 2 ? 3 : 1 ); i2++ )
+                {
+                    int32_t idx = T[0] + i0 + T[1] + 5 * i1;
+                    if constexpr( D > 2 ) idx += T[D - 1] + 25 * i2;
+                    const CellEntry cell = tbl[idx];
+                    const float invMag = cp.jitter * cell.invMag;
+                    const int ii[3] = { i0, i1, i2 };
+                    float local[D], d[D];
+#pragma unroll
+                    for( int a = 0; a < D; a++ ) { local[a] = fmadd( cell.v[a], invMag, F[a][ii[a]] ); d[a] = local[a] - pos[a]; }
+                    float nd = calcDistance<D>( cp.distFunc, false, d, cp.minkowskiP );
+                    if( cp.sizeJitterOn ) nd *= fnmadd( cp.sizeJitter, sizeJitterBits<D>( htbl[idx] ), 1.0f );
+                    bool closer = nd < distance;
+                    distance = fminf_( nd, distance );
+#pragma unroll
+                    for( int a = 0; a < D; a++ ) cellPos[a] = closer ? local[a] : cellPos[a];
+                }
+        return true;
+    }
+
     // CellularValue (Cellular.inl:17-292) on the wave's cell table: the
     // reference's walk, x outermost, over the same cells with the same
     // running offsets, since the value of tied distances depends on the
@@ -790,6 +894,10 @@ namespace fn2d
                 return;
             }
         }
+#if defined( __HIP_DEVICE_COMPILE__ )
+        if constexpr( TABLE && ( D == 2 || D == 3 ) )
+            if( cellularDistancesTableAll<D, SLOTS>( cp, pos, dist ) ) return;
+#endif
         dist[0] = dist[1] = dist[2] = dist[3] = __builtin_inff();
         cellWalk<D, false>( cp, pos, [&]( int32_t h, const float* v, float invMag, const float* fc ) {
             float d[D];

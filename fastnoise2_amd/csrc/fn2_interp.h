@@ -242,7 +242,13 @@ namespace fn2i
         if constexpr( opIn<OP, fn2::OP_CELL_LOOKUP>() )
         {
             float cell[D];
-            cellularLookupPos<D>( cp, pos, cell );
+#if defined( __HIP_DEVICE_COMPILE__ )
+            bool done = false;
+            if constexpr( ( OP >= 0 ) && FN2_CELL_TABLE && !FN2_SHARED_CELL_WALK && ( D == 2 || D == 3 ) )
+                done = cellularLookupTable<D>( cp, pos, cell );
+            if( !done )
+#endif
+                cellularLookupPos<D>( cp, pos, cell );
             const float scale = K[5];
 #pragma unroll
             for( int a = 0; a < D; a++ ) R[in.r[0] + a] = cell[a] * scale;

@@ -83,20 +83,34 @@ def test_huge_and_special_coordinates(gpu):
     assert_bitexact(out, ref, "huge and special coordinates")
 
 
+TABLE_NODES = [
+    ("CellularValue", [dict(value_index=0, distance_function="Euclidean"), dict(value_index=2, distance_function="Manhattan"),
+                       dict(value_index=1, distance_function="Hybrid")]),
+    ("CellularDistance", [dict(distance_function="Manhattan", return_type="Index0Add1", distance_index_0=0, distance_index_1=1),
+                          dict(distance_function="Hybrid", return_type="Index0Sub1", distance_index_0=2, distance_index_1=0),
+                          dict(distance_function="Euclidean", return_type="Index0Div1", distance_index_0=0, distance_index_1=3)]),
+    ("CellularLookup", [dict(distance_function="Euclidean", lookup=S("Simplex", feature_scale=3.0)),
+                        dict(distance_function="Manhattan", lookup=S("White"))]),
+]
+
+
+@pytest.mark.parametrize("kind", range(len(TABLE_NODES)), ids=[k for k, _ in TABLE_NODES])
 @pytest.mark.parametrize("dims", [2, 3])
-@pytest.mark.parametrize("jitter", range(len(JITTERS)), ids=["j1", "jneg", "j0", "jsmall", "jbig", "jnode"])
-def test_value_table_walk_matches_oracle(gpu, dims, jitter):
-    # CellularValue in a specialised kernel reads each cell's hash, jitter
-    # vector and inverse length from the wave's LDS table when the wave's
-    # cells lie close together (grids, clustered positions), and walks in
-    # the reference's order, which decides between tied distances (jitter 0);
+@pytest.mark.parametrize("jitter", [0, 2, 5], ids=["j1", "j0", "jnode"])
+def test_table_walks_match_oracle(gpu, kind, dims, jitter):
+    # In a specialised kernel, CellularValue, CellularLookup and the
+    # CellularDistance cases the pruned walk leaves (other distance
+    # functions, size jitter) read each cell's hash, jitter vector and
+    # inverse length from the wave's LDS table when the wave's cells lie
+    # close together (grids, clustered positions), walking in the
+    # reference's order, which decides between tied distances (jitter 0);
     # scattered positions take the per-lane walk. Per-point jitter and size
     # jitter come from nodes, so lanes of one wave scale the same table
     # entries differently.
+    name, variants = TABLE_NODES[kind]
     size_jitter = S("Value", feature_scale=5.0) if jitter == 5 else 0.25
-    for vi, func in [(0, "Euclidean"), (2, "Manhattan"), (1, "Hybrid")]:
-        spec = S("CellularValue", feature_scale=9.0, distance_function=func, value_index=vi,
-                 grid_jitter=JITTERS[jitter], size_jitter=size_jitter)
+    for members in variants:
+        spec = S(name, feature_scale=9.0, grid_jitter=JITTERS[jitter], size_jitter=size_jitter, **members)
         node = fn.Node.from_encoded(spec.encode())
         cnt = {2: [97, 41], 3: [61, 13, 10]}[dims]
         off = [-31.25, 7.5, 100.0][:dims]
@@ -104,7 +118,8 @@ def test_value_table_walk_matches_oracle(gpu, dims, jitter):
         out, mm = gen(*off, *cnt, *([0.37] * dims), seed=777)
         assert node.jit_state(dims) == 1, fn.last_error()
         ref, ref_mm = oracle.Tree(spec.encode()).grid(off, cnt, [0.37] * dims, seed=777)
-        assert_bitexact(out, ref, "%dD grid jitter %d value index %d %s" % (dims, jitter, vi, func))
+        what = "%s %dD jitter %d %s" % (name, dims, jitter, members.get("distance_function"))
+        assert_bitexact(out, ref, what + " grid")
         assert mm == ref_mm
         rng = np.random.default_rng(dims + 10 * jitter)
         pos = rng.uniform(-300.0, 300.0, (dims, 3000)).astype(np.float32)
@@ -112,5 +127,5 @@ def test_value_table_walk_matches_oracle(gpu, dims, jitter):
         pos[:, 500:600] = (np.round(pos[:, 500:600] / 9.0) * 9.0).astype(np.float32)          # on lattice points
         out, mm = node.gen_position_array(list(pos), seed=777)
         ref, ref_mm = oracle.Tree(spec.encode()).positions(list(pos), seed=777)
-        assert_bitexact(out, ref, "%dD positions jitter %d value index %d %s" % (dims, jitter, vi, func))
+        assert_bitexact(out, ref, what + " positions")
         assert mm == ref_mm
