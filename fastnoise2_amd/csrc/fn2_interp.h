@@ -263,7 +263,10 @@ namespace fn2i
         for( int a = 0; a < D; a++ ) { sp[a] = R[in.r[1] + a]; out[a] = R[in.r[2] + a]; axis[a] = K[a]; }
         const float amp = R[in.r[3]];
         float s = 0.0f;
-        if constexpr( OP == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );
+        // specialised kernels of grid-like requests try the wave-uniform
+        // corner loop; the position-array variant and the interpreter skip
+        // the check, which scattered points never pass
+        if constexpr( OP == fn2::OP_WARP_GRADIENT ) s = warpGradient<D, !FN2_SHARED_CELL_WALK>( seed, amp, axis, sp, out );
         else if constexpr( OP == fn2::OP_WARP_SIMPLEX ) s = warpSimplex<D>( seed, in.p[0], amp, axis, sp, out );
         else if constexpr( OP == fn2::OP_WARP_SUPERSIMPLEX ) s = warpSuperSimplex<D>( seed, in.p[0], amp, axis, sp, out );
         else if( in.op == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );
