@@ -122,7 +122,13 @@ namespace fn2d
     // ---------------- Simplex (Simplex.inl:7-286) ----------------
     // Returns the un-scaled value; the three falloffs and gradients are combined
     // in the reference's order for each dimension.
-    template<bool G = true>
+    // U: try the wave-uniform cell first. When every active lane of a wave
+    // lies in the same skewed lattice cell (grid bricks at low octaves), the
+    // cell's vertex hashes are the same on every lane: they are computed from
+    // the first lane's cell with plain xors, which the compiler keeps on the
+    // scalar unit, and each lane only picks its middle vertex. The values
+    // are those of the per-lane path.
+    template<bool G = true, bool U = false>
     FN2D float simplex2( int32_t seed, float x, float y )
     {
         constexpr double kSkew2 = 1.0 / ( kRoot3 + 1.0 );
@@ -131,7 +137,7 @@ namespace fn2d
         float xSk = x + skewDelta, ySk = y + skewDelta;
         float xB = ffloor( xSk ), yB = ffloor( ySk );
         float dxS = xSk - xB, dyS = ySk - yB;
-        int32_t xP = imul( floorToInt<G>( xB ), kPX ), yP = imul( floorToInt<G>( yB ), kPY );
+        const int32_t xi = floorToInt<G>( xB ), yi = floorToInt<G>( yB );
         bool xGEy = dxS >= dyS;
         float unskewDelta = (float)kUnskew2 * ( dxS + dyS );
         float dx0 = dxS + unskewDelta, dy0 = dyS + unskewDelta;
@@ -144,6 +150,25 @@ namespace fn2d
         float f2 = f0 + fmadd( unskewDelta, (float)( -4.0 * ( kRoot3 + 2.0 ) / ( kRoot3 + 3.0 ) ), (float)( -2.0 / 3.0 ) );
         f0 = fmaxf_( f0, 0.0f ); f1 = fmaxf_( f1, 0.0f ); f2 = fmaxf_( f2, 0.0f );
         f0 *= f0; f0 *= f0; f1 *= f1; f1 *= f1; f2 *= f2; f2 *= f2;
+#if defined( __HIP_DEVICE_COMPILE__ )
+        if constexpr( U )
+        {
+            const int32_t xu = __builtin_amdgcn_readfirstlane( xi ), yu = __builtin_amdgcn_readfirstlane( yi );
+            const int32_t su = __builtin_amdgcn_readfirstlane( seed );
+            if( waveAll( xi == xu && yi == yu && seed == su ) )
+            {
+                const int32_t xP = imul( xu, kPX ), yP = imul( yu, kPY );
+                const int32_t x1 = iadd( xP, kPX ), y1 = iadd( yP, kPY );
+                const int32_t h0 = hashFinal( su ^ xP ^ yP ), h2 = hashFinal( su ^ x1 ^ y1 );
+                const int32_t hX = hashFinal( su ^ x1 ^ yP ), hY = hashFinal( su ^ xP ^ y1 );
+                float g0 = gradDot2( h0, dx0, dy0 );
+                float g1 = gradDot2( xGEy ? hX : hY, dx1, dy1 );
+                float g2 = gradDot2( h2, dx2, dy2 );
+                return fmadd( g0, f0, fmadd( g1, f1, g2 * f2 ) );
+            }
+        }
+#endif
+        int32_t xP = imul( xi, kPX ), yP = imul( yi, kPY );
         float g0 = gradDot2( hash2( seed, xP, yP ), dx0, dy0 );
         float g1 = gradDot2( hash2( seed, xGEy ? iadd( xP, kPX ) : xP, xGEy ? yP : iadd( yP, kPY ) ), dx1, dy1 );
         float g2 = gradDot2( hash2( seed, iadd( xP, kPX ), iadd( yP, kPY ) ), dx2, dy2 );

@@ -22,6 +22,11 @@
 // walk. The position-array variant with the shared far-cell walk leaves the
 // table out: its points are scattered, and the two LDS regions together
 // would cost resident blocks.
+// Specialised fused fractals try the wave-uniform lattice cell of their
+// leaf (simplex2<G, true>); defined 0, they take the per-lane path only
+#ifndef FN2_UNIFORM_LATTICE
+#define FN2_UNIFORM_LATTICE 1
+#endif
 #ifndef FN2_CELL_TABLE
 #define FN2_CELL_TABLE 1
 #endif
@@ -70,7 +75,7 @@ namespace fn2i
     // Lattice noise on explicit coordinates; LEAF as OP above. G = false
     // leaves out the out-of-range select of the float -> int conversions, for
     // coordinates shown to lie below kUnguardedCoord in magnitude.
-    template<int D, int LEAF = -1, bool G = true>
+    template<int D, int LEAF = -1, bool G = true, bool U = false>
     FN2D float leafRaw( int op, int32_t seed, const float* x )
     {
         if constexpr( opIn<LEAF, fn2::OP_PERLIN>() )
@@ -90,7 +95,7 @@ namespace fn2i
         if constexpr( opIn<LEAF, fn2::OP_SIMPLEX>() )
             if( LEAF >= 0 || op == fn2::OP_SIMPLEX )
             {
-                if constexpr( D == 2 ) return simplex2<G>( seed, x[0], x[1] );
+                if constexpr( D == 2 ) return simplex2<G, U>( seed, x[0], x[1] );
                 else if constexpr( D == 3 ) return simplex3<G>( seed, x[0], x[1], x[2] );
                 else return simplex4<G>( seed, x[0], x[1], x[2], x[3] );
             }
@@ -132,7 +137,8 @@ namespace fn2i
             float x[D];
 #pragma unroll
             for( int a = 0; a < D; a++ ) x[a] = pos[a] * freq;
-            return scaleOutput( leafRaw<D, LEAF, G>( leaf, s, x ), nMin, oScale, rMin );
+            // specialised kernels try the wave-uniform lattice cell
+            return scaleOutput( leafRaw<D, LEAF, G, ( LEAF >= 0 ) && FN2_UNIFORM_LATTICE>( leaf, s, x ), nMin, oScale, rMin );
         };
         float amp = 1.0f;
         float noise = octave( seed );
