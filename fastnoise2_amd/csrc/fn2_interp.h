@@ -123,7 +123,7 @@ namespace fn2i
     // A specialised kernel knows the octave count; FN2_OCTAVE_UNROLL (0: the
     // compiler's choice) sets how far the loop is unrolled, a per-tree tuning
     // knob of the serving engine.
-    template<int D, int LEAF, bool G>
+    template<int D, int LEAF, bool G, bool BRICK>
     FN2D float fractalOctaves( const Ins& in, const float* K, int32_t seed, const float* start )
     {
         const int leaf = in.p[0], octaves = in.p[1];
@@ -137,8 +137,10 @@ namespace fn2i
             float x[D];
 #pragma unroll
             for( int a = 0; a < D; a++ ) x[a] = pos[a] * freq;
-            // specialised kernels try the wave-uniform lattice cell
-            return scaleOutput( leafRaw<D, LEAF, G, ( LEAF >= 0 ) && FN2_UNIFORM_LATTICE>( leaf, s, x ), nMin, oScale, rMin );
+            // specialised kernels walking grid bricks try the wave-uniform
+            // lattice cell (rows of 64 rarely share one, and then pay the
+            // check for nothing)
+            return scaleOutput( leafRaw<D, LEAF, G, BRICK && ( LEAF >= 0 ) && FN2_UNIFORM_LATTICE>( leaf, s, x ), nMin, oScale, rMin );
         };
         float amp = 1.0f;
         float noise = octave( seed );
@@ -177,7 +179,7 @@ namespace fn2i
     // far below 2^31, where the float -> int conversion needs no select
     constexpr float kUnguardedCoord = 268435456.0f;   // 2^28
 
-    template<int D, int LEAF = -1, class RF = const float*>
+    template<int D, int LEAF = -1, bool BRICK = false, class RF = const float*>
     FN2D float fractalLeaf( const Ins& in, const float* K, int32_t seed, RF R, int p )
     {
         float pos[D];
@@ -198,9 +200,9 @@ namespace fn2i
             bool small = true;
 #pragma unroll
             for( int a = 0; a < D; a++ ) small = small && fabs_( pos[a] ) < limit;
-            if( waveAll( small ) ) return fractalOctaves<D, LEAF, false>( in, K, seed, pos );
+            if( waveAll( small ) ) return fractalOctaves<D, LEAF, false, BRICK>( in, K, seed, pos );
         }
-        return fractalOctaves<D, LEAF, true>( in, K, seed, pos );
+        return fractalOctaves<D, LEAF, true, BRICK>( in, K, seed, pos );
     }
 
     // SLOTS: distance slots CellularDistance keeps (4, or fewer when the
@@ -304,7 +306,9 @@ namespace fn2i
     // instruction's op and, for OP_FRACTAL_LEAF, its leaf op, for
     // OP_CELL_DISTANCE the number of distance slots it reads, so that only
     // the code the instruction runs is instantiated.
-    template<int DM, int FT, int OP = -1, int AUX = -1, class RF = float*>
+    // BRICK: the point is one lane of a wave-sized grid brick (specialised
+    // kernels walking a grid in bricks)
+    template<int DM, int FT, int OP = -1, int AUX = -1, bool BRICK = false, class RF = float*>
     FN2D void execIns( const Ins& in, const float* K, const int32_t seed, RF R )
     {
         {
@@ -325,9 +329,9 @@ namespace fn2i
                 if constexpr( opIn<OP, fn2::OP_FRACTAL_LEAF>() )
                 {
                     float v = 0.0f;
-                    if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2, AUX>( in, K, seed, R, in.r[1] );
-                    else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3, AUX>( in, K, seed, R, in.r[1] );
-                    else if( DM & 16 ) v = fractalLeaf<4, AUX>( in, K, seed, R, in.r[1] );
+                    if( ( DM & 4 ) && d == 2 ) v = fractalLeaf<2, AUX, BRICK>( in, K, seed, R, in.r[1] );
+                    else if( ( DM & 8 ) && d == 3 ) v = fractalLeaf<3, AUX, BRICK>( in, K, seed, R, in.r[1] );
+                    else if( DM & 16 ) v = fractalLeaf<4, AUX, BRICK>( in, K, seed, R, in.r[1] );
                     R[in.r[0]] = v;
                 }
                 break;

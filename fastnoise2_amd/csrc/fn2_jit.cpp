@@ -136,7 +136,8 @@ namespace fn2
                  "static constexpr bool kFresh = true;\n"
                  "static constexpr int kDims = " << p.dims << ";\n"
                  "static constexpr bool kBricks = " << ( PrunedCellular( p ) || UniformSimplex2D( p ) ? "true" : "false" ) << ";\n"
-                 "__device__ __forceinline__ float operator()( const fn2::GenArgs& a, float* R ) const {\n";
+                 "template<bool BRICK = false>\n"
+                 "__device__ __forceinline__ float eval( const fn2::GenArgs& a, float* R ) const {\n";
             // long programs get a register fence between instructions: measured
             // on MI355X, a 355-instruction tree needs 256 VGPRs without (one wave
             // per SIMD, 2.46 ms on 128^3) and 92 with (5 waves, 1.63 ms); a
@@ -162,9 +163,11 @@ namespace fn2
                 }
                 s << "  execIns<" << ( in.dim >= 2 && in.dim <= 4 ? 1 << in.dim : 4 ) << ", " << OpFeature( in.op ) << ", "
                   << (int)in.op << ", " << Aux( in )
-                  << ">( in, k, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
+                  << ", BRICK>( in, k, iadd( a.seed, (int32_t)" << (uint32_t)in.seed << "u ), R ); }\n";
             }
-            s << "return R[" << p.resultReg << "];\n} };\n}\n"
+            s << "return R[" << p.resultReg << "];\n}\n"
+                 "__device__ __forceinline__ float operator()( const fn2::GenArgs& a, float* R ) const { return eval<false>( a, R ); }\n"
+                 "};\n}\n"
                  "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_single( fn2::GenArgs a )\n"
                  "{ fn2k::generateSingle<256>( a, Prog() ); }\n"
                  "extern \"C\" __global__ void __launch_bounds__( 256 ) fn2_jit_batch( const fn2::GenArgs* __restrict__ jobs, "
