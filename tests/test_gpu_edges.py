@@ -182,3 +182,27 @@ def test_fused_fractal_special_positions(gpu, kind, ws, jit):
     ref, ref_mm = oracle.Tree(spec.encode()).positions([xs, ys], seed=11)
     assert_bitexact(out, ref, "%s ws %g special positions" % (kind, ws))
     assert mm == ref_mm or (np.isnan(mm[0]) and np.isnan(ref_mm[0]))
+
+
+@pytest.mark.parametrize("kind", ["FractalFBm", "FractalRidged"])
+def test_fused_simplex_bricks_on_ragged_grids(gpu, kind):
+    # a specialised fused 2D Simplex fractal walks grids in 8 x 8 bricks and
+    # hashes the lattice cell once per wave when all lanes share it; ragged
+    # sizes leave partial bricks, and offsets from small to 1e7 move waves
+    # between the shared-cell and per-lane paths and past the coordinate
+    # bound of the conversion select
+    spec = S(kind, source=S("Simplex", feature_scale=37.0), octaves=5, gain=0.6, lacunarity=2.0)
+    node = fn.Node.from_encoded(spec.encode())
+    tree = oracle.Tree(spec.encode())
+    old = fn.set_jit_mode(fn.JIT_SYNC)
+    try:
+        for cnt in ([97, 41], [8, 8], [1, 300], [300, 1], [13, 5], [64, 9]):
+            for off in ([0.0, 0.0], [-1.0e4, 3.0e4], [1.0e7, -1.0e7], [-3.5, 2.25]):
+                for step in (1.0, 0.37):
+                    out, mm = node.gen_uniform_grid_2d(off[0], off[1], cnt[0], cnt[1], step, step, seed=-4321)
+                    assert node.jit_state(2) == 1, fn.last_error()
+                    ref, ref_mm = tree.grid(off, cnt, [step, step], seed=-4321)
+                    assert_bitexact(out, ref, "%s %s at %s step %g" % (kind, cnt, off, step))
+                    assert mm == ref_mm
+    finally:
+        fn.set_jit_mode(old)
