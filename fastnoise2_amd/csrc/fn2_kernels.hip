@@ -153,12 +153,13 @@ namespace
     }
 
     // Blocks of a single-call (grid-stride) kernel. With the kernel's resident
-    // blocks per CU known: 4 rounds of what the GPU holds at once, and at
-    // least 2 points per thread (measured on MI355X with the range record's
-    // spread slots: 4 rounds 2-5 % faster than 2 on the cellular and serving
-    // trees, the same on FBm; 1 or 2 points per thread the same on large
-    // calls, and 1 slower on the 262 k-point Perlin grid: 5.0 against
-    // 4.5 us). Without it, 16 blocks per CU.
+    // blocks per CU known: up to 4 rounds of what the GPU holds at once, one
+    // point per thread below that (measured on MI355X: 4 rounds 2-5 % faster
+    // than 2 on the cellular and serving trees, the same on FBm; with the
+    // 64-slot range record, one point per thread against at least two: the
+    // 1 M-point tileable call 0.0332 -> 0.0327 ms, the 262 k-point Perlin
+    // grid and large calls unchanged; at least four: tileable 0.0344 ms,
+    // Perlin 4.9 against 4.4 us). Without it, 16 blocks per CU.
     unsigned GridFor( int64_t total, int blocksPerCU = 0, int rounds = 4 )
     {
         if( blocksPerCU <= 0 )
@@ -167,7 +168,7 @@ namespace
             const int64_t cap = (int64_t)CUs() * 16;
             return (unsigned)( need < cap ? need : cap );
         }
-        const int64_t need = std::max<int64_t>( 1, ( total + 2 * kBlock - 1 ) / ( 2 * kBlock ) );
+        const int64_t need = std::max<int64_t>( 1, ( total + kBlock - 1 ) / kBlock );
         const int64_t cap = (int64_t)CUs() * blocksPerCU * rounds;
         return (unsigned)( need < cap ? need : cap );
     }
