@@ -22,6 +22,11 @@
 // walk. The position-array variant with the shared far-cell walk leaves the
 // table out: its points are scattered, and the two LDS regions together
 // would cost resident blocks.
+// Specialised gradient warps try the wave-uniform lattice cell (defined 0:
+// the per-lane corner loop only; long programs' kernels do)
+#ifndef FN2_UNIFORM_WARP
+#define FN2_UNIFORM_WARP 1
+#endif
 // Specialised fused fractals try the wave-uniform lattice cell of their
 // leaf (simplex2<G, true>); defined 0, they take the per-lane path only
 #ifndef FN2_UNIFORM_LATTICE
@@ -274,7 +279,7 @@ namespace fn2i
         // specialised kernels of grid-like requests try the wave-uniform
         // corner loop; the position-array variant and the interpreter skip
         // the check, which scattered points never pass
-        if constexpr( OP == fn2::OP_WARP_GRADIENT ) s = warpGradient<D, !FN2_SHARED_CELL_WALK>( seed, amp, axis, sp, out );
+        if constexpr( OP == fn2::OP_WARP_GRADIENT ) s = warpGradient<D, !FN2_SHARED_CELL_WALK && FN2_UNIFORM_WARP>( seed, amp, axis, sp, out );
         else if constexpr( OP == fn2::OP_WARP_SIMPLEX ) s = warpSimplex<D>( seed, in.p[0], amp, axis, sp, out );
         else if constexpr( OP == fn2::OP_WARP_SUPERSIMPLEX ) s = warpSuperSimplex<D>( seed, in.p[0], amp, axis, sp, out );
         else if( in.op == fn2::OP_WARP_GRADIENT ) s = warpGradient<D>( seed, amp, axis, sp, out );

@@ -103,6 +103,9 @@ namespace fn2
         // Whether the program holds a CellularDistance that takes the pruned
         // cell walk (squared distance, no size jitter): such programs walk grid
         // requests in bricks, which lets whole waves leave out more cells
+        // programs of at least this many instructions get the lean source
+        constexpr size_t kJitLeanMin = 128;
+
         bool PrunedCellular( const DeviceProgram& p )
         {
             for( const Ins& in : p.ins )
@@ -130,12 +133,21 @@ namespace fn2
             std::ostringstream s;
             if( variant & kJitUnrollMask ) s << "#define FN2_OCTAVE_UNROLL " << kJitUnrollFactor[variant & kJitUnrollMask] << "\n";
             if( variant & kJitSharedWalk ) s << "#define FN2_SHARED_CELL_WALK 1\n";
+            // long programs leave out the code that is duplicated per
+            // instruction for coherent waves (cell tables, wave-uniform warp
+            // cells, a brick-specific program copy): measured with hipcc at
+            // -O3, the 355-instruction mixed tree compiled in 73 s without
+            // them and 210 s with them, which a serving engine's first
+            // requests would wait out on the interpreter
+            const bool lean = p.ins.size() >= (size_t)kJitLeanMin;
+            if( lean ) s << "#define FN2_CELL_TABLE 0\n#define FN2_UNIFORM_WARP 0\n";
             s << "#include \"fn2_kernel_body.h\"\n"
                  "using namespace fn2d;\nusing namespace fn2i;\n"
                  "namespace {\nstruct Prog {\n"
                  "static constexpr bool kFresh = true;\n"
                  "static constexpr int kDims = " << p.dims << ";\n"
                  "static constexpr bool kBricks = " << ( PrunedCellular( p ) || UniformSimplex2D( p ) ? "true" : "false" ) << ";\n"
+                 "static constexpr bool kBrickAware = " << ( lean ? "false" : "true" ) << ";\n"
                  "template<bool BRICK = false>\n"
                  "__device__ __forceinline__ float eval( const fn2::GenArgs& a, float* R ) const {\n";
             // long programs get a register fence between instructions: measured

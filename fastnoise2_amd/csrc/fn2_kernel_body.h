@@ -123,8 +123,11 @@ namespace fn2k
                 fresh.R[0] = fmadd( (float)(int32_t)x, a.step[0], a.offset[0] );
                 fresh.R[1] = fmadd( (float)(int32_t)y, a.step[1], a.offset[1] );
                 if constexpr( D == 3 ) fresh.R[2] = fmadd( (float)(int32_t)z, a.step[2], a.offset[2] );
-                // the program knows it runs in bricks (fn2i::fractalLeaf)
-                const float v = prog.template eval<true>( a, fresh.R );
+                // the program knows it runs in bricks (fn2i::fractalLeaf),
+                // unless it is one copy for both walks (long programs)
+                float v;
+                if constexpr( Prog::kBrickAware ) v = prog.template eval<true>( a, fresh.R );
+                else v = prog( a, fresh.R );
                 // a plain store: a wave writes 16-byte pieces of 16 lines
                 // here, which the L2 merges with its neighbours' pieces;
                 // streaming stores went out piece by piece (2.8 times the
